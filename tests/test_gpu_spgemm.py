@@ -1,0 +1,120 @@
+"""GPU thresholded SpGEMM, filter, union axpby and McWeeny purification
+against the reference's golden outputs and the oracle.  FP64 values 1e-12
+relative; patterns identical (the estimates differ from the reference's only
+in summation order, so a block may flip only when its estimate lies within
+rounding of the threshold -- none does in these cases).  Reduced-precision
+numeric phases (FP32, BF16 on the matrix cores) keep the FP64 symbolic
+pattern; their values are checked to 1e-5 (FP32) and 2e-2 (BF16) relative."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def from_golden(d, p):
+    from vbcsr_amd import VBCSR
+
+    return VBCSR.from_block_csr(d[f"{p}_bs"], d[f"{p}_row_ptr"], d[f"{p}_col_ind"], d[f"{p}_vals"])
+
+
+@pytest.mark.parametrize("tag", ["t0", "t1", "t2"])
+def test_golden_spgemm(golden_real, tag):
+    d = golden_real
+    A = from_golden(d, "A")
+    C = A.spmm(A, float(d[f"C{tag}_thr"]))
+    assert np.array_equal(C.row_ptr, d[f"C{tag}_row_ptr"])
+    assert np.array_equal(C.col_ind, d[f"C{tag}_col_ind"])
+    assert relerr(C.get_values(), d[f"C{tag}_vals"]) < 1e-12
+
+
+def test_golden_spgemm_complex(golden_complex):
+    d = golden_complex
+    G = from_golden(d, "G")
+    C = G @ G
+    assert np.array_equal(C.row_ptr, d["C_row_ptr"]) and np.array_equal(C.col_ind, d["C_col_ind"])
+    assert relerr(C.get_values(), d["C_vals"]) < 1e-12
+
+
+def test_golden_filter_axpby_shift(golden_real):
+    d = golden_real
+    A = from_golden(d, "A")
+    F = A.duplicate()
+    F.filter_blocks(float(d["F_thr"]))
+    assert np.array_equal(F.row_ptr, d["F_row_ptr"]) and np.array_equal(F.col_ind, d["F_col_ind"])
+    assert np.array_equal(F.get_values(), d["F_vals"])
+    C1 = A.spmm(A, float(d["Ct1_thr"]))
+    C1.axpby(-2.0, A, 3.0)
+    assert np.array_equal(C1.row_ptr, d["U_row_ptr"]) and np.array_equal(C1.col_ind, d["U_col_ind"])
+    assert relerr(C1.get_values(), d["U_vals"]) < 1e-12
+    S = A.duplicate()
+    S.shift(0.75)
+    assert np.array_equal(S.get_values(), d["S_vals"])
+
+
+def test_config1_spgemm_vs_oracle():
+    from vbcsr_amd import VBCSR
+    from vbcsr_amd import workloads as W
+
+    A, _ = W.config1_spmv()
+    H = VBCSR.from_block_csr(A.bs, A.row_ptr, A.col_ind, A.vals)
+    m = O.Mat.of(A)
+    norms = O.block_norms(m)
+    for thr in (0.0, float(np.median(norms)) * 4, float(norms.max()) * 3):
+        C = H.spmm(H, thr)
+        Cr = O.spgemm(m, m, thr)
+        assert np.array_equal(C.row_ptr, Cr.row_ptr) and np.array_equal(C.col_ind, Cr.col_ind), thr
+        assert relerr(C.get_values(), Cr.vals) < 1e-12
+
+
+def test_asymmetric_row_eps():
+    from vbcsr_amd import VBCSR
+
+    adj = [[0, 1, 2], [0, 1, 3], [0, 2, 3], [1, 2, 3, 4], [3, 4]]
+    val = {(0, 0): 1, (0, 1): 0.8, (0, 2): 0.2, (1, 0): 0.8, (1, 1): 1, (1, 3): 1, (2, 0): 0.2, (2, 2): 1,
+           (2, 3): 1, (3, 1): 1, (3, 2): 1, (3, 3): 1, (3, 4): 1, (4, 3): 1, (4, 4): 1}
+    H = VBCSR.create_serial(5, [1] * 5, adj)
+    for (i, j), v in val.items():
+        H.add_block(i, j, np.array([[float(v)]]))
+    H.assemble()
+    D = H.spmm(H, 0.9).to_dense()
+    assert abs(D[0, 3] - D[3, 0]) < 1e-12
+
+
+@pytest.mark.parametrize("numeric,tol", [("fp32", 1e-5), ("bf16", 2e-2)])
+def test_reduced_precision_numeric(numeric, tol):
+    from vbcsr_amd import VBCSR
+    from vbcsr_amd import workloads as W
+
+    P = W.mcweeny_start(nb=300, neighbours=12.0, seed=2)
+    H = VBCSR.from_block_csr(P.bs, P.row_ptr, P.col_ind, P.vals)
+    m = O.Mat.of(P)
+    thr = 1e-5
+    C = H.spmm(H, 0.0, numeric=numeric)
+    Cr = O.spgemm(m, m, 0.0)
+    assert np.array_equal(C.row_ptr, Cr.row_ptr) and np.array_equal(C.col_ind, Cr.col_ind)
+    assert relerr(C.get_values(), Cr.vals) < tol
+    C2 = H.spmm(H, thr, numeric=numeric)
+    assert C2.local_block_nnz <= Cr.col_ind.shape[0]
+
+
+def test_mcweeny_iterations_vs_oracle():
+    """P <- 3 P^2 - 2 P^3 with thresholded products (README.md:21-34 form)."""
+    from vbcsr_amd import VBCSR
+    from vbcsr_amd import workloads as W
+
+    P0 = W.mcweeny_start(nb=400, neighbours=14.0, seed=1)
+    thr = 1e-5
+    P = VBCSR.from_block_csr(P0.bs, P0.row_ptr, P0.col_ind, P0.vals)
+    pm = O.Mat.of(P0)
+    for it in range(3):
+        P2 = P.spmm(P, thr)
+        P = 3.0 * P2 - 2.0 * P2.spmm(P, thr)
+        p2 = O.spgemm(pm, pm, thr)
+        p3 = O.spgemm(p2, pm, thr)
+        pm = O.axpby(p2, -2.0, p3, 3.0)
+        assert np.array_equal(P.row_ptr, pm.row_ptr) and np.array_equal(P.col_ind, pm.col_ind), it
+        assert relerr(P.get_values(), pm.vals) < 1e-11, it
+    assert abs(P.trace() - np.trace(W.BlockCSR(P0.bs, pm.row_ptr, pm.col_ind, pm.vals).to_dense())) < 1e-8

@@ -1,0 +1,258 @@
+// Block SpMV / SpMM and their adjoints on gfx950.
+//
+// One wavefront owns one block row at a time.  Its 64 lanes are split into
+//   v  (fastest)  : Kp = min(64, pow2ceil(K)) vector columns,
+//   a             : scalar rows of the block row handled in this pass,
+//   p  (slowest)  : P partitions of the row's blocks (block k goes to
+//                   partition (k - row_start) mod P),
+// so a lane accumulates sum_k sum_b A_k[a][b] X[col_k + b][v] over its
+// partition; the P partial sums are merged with cross-lane xor shuffles.  A
+// block's row a is read as one contiguous run (vectorised when the block size
+// is a compile-time constant), X rows are read coalesced across the v lanes,
+// and every Y element is written exactly once -- no atomics, no Y zero pass.
+// Work-groups take work-balanced contiguous chunks of block rows (weights
+// sum r*c per row), remapped so that each XCD streams one contiguous run of
+// rows and keeps the X rows it gathers in its own L2.
+//
+// The optional fused epilogue computes Y = alpha*AX + beta*X_own + gamma*Z and
+// FP64 per-column partial dot products of Y with W (or Y) and with X_own, one
+// partial per work-group, written (not accumulated) into partials[g][2K].
+#include "apply.hpp"
+
+namespace vbh {
+
+template <typename T, int BS, bool FUSED>
+__global__ __launch_bounds__(kThreads) void k_apply_rows(ApplyArgs<T> g) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int chunk = xcd_chunk(g.G);
+    double d1 = 0.0, d2 = 0.0;
+    row_loop<T, BS, FUSED>(g, g.split[chunk], g.split[chunk + 1], lane, wave, d1, d2);
+    if constexpr (FUSED) {
+        if (g.partials != nullptr) write_partials(g.K, lane, wave, d1, d2, g.partials + int64_t(chunk) * 2 * g.K);
+    }
+}
+
+// Adjoint: output block column j = sum over incoming (row i, slot k) of
+// A_k^H X_i; the lane map is the forward one with the column's incoming list
+// in place of the row's blocks and the block read down a column.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_apply_adjoint(AdjointArgs<T> g) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int chunk = xcd_chunk(g.G);
+    const int c0 = g.split[chunk], c1 = g.split[chunk + 1];
+    const DevPattern& p = g.p;
+    const int K = g.K;
+    const int Kp = K >= 64 ? 64 : pow2ceil(K);
+    const int lgK = __builtin_ctz(Kp);
+    const int v_l = lane & (Kp - 1);
+    for (int j = c0 + wave; j < c1; j += kWaves) {
+        const int c = p.cbs[j];
+        const int qb = g.t_ptr[j], qe = g.t_ptr[j + 1];
+        const int Ar = 64 >> lgK;
+        const int cp = pow2ceil(c);
+        const int cpp = cp <= Ar ? cp : Ar;
+        const int P = Ar / cpp;
+        const int lgC = __builtin_ctz(cpp);
+        const int b_l = (lane >> lgK) & (cpp - 1);
+        const int p_l = lane >> (lgK + lgC);
+        for (int b0 = 0; b0 < c; b0 += cpp) {
+            const int b = b0 + b_l;
+            for (int v0 = 0; v0 < K; v0 += Kp) {
+                const int v = v0 + v_l;
+                const bool active = b < c && v < K;
+                T acc = Sc<T>::zero();
+                if (active) {
+                    for (int q = qb + p_l; q < qe; q += P) {
+                        const int i = g.t_row[q];
+                        const int k = g.t_slot[q];
+                        const int r = p.rbs[i];
+                        const T* blk = g.vals + p.voff[k] + b;
+                        const T* xr = g.X + int64_t(p.roff[i]) * g.ldx + v;
+                        for (int a = 0; a < r; ++a)
+                            acc = Sc<T>::fma(Sc<T>::conj(blk[int64_t(a) * c]), xr[int64_t(a) * g.ldx], acc);
+                    }
+                }
+                for (int s = Kp * cpp; s < 64; s <<= 1) acc = acc + shfl_xor(acc, s);
+                if (active && p_l == 0) g.Y[(int64_t(p.coff[j]) + b) * g.ldy + v] = acc;
+            }
+        }
+    }
+}
+
+// Sum per-work-group partials: out[e] (+)= sum_g partials[g][e], e < E.
+__global__ void k_reduce_partials(const double* __restrict__ partials, int G, int E, double* __restrict__ out,
+                                  int accumulate) {
+    const int e = blockIdx.x;
+    double s = 0.0;
+    for (int gg = threadIdx.x; gg < G; gg += blockDim.x) s += partials[int64_t(gg) * E + e];
+    __shared__ double red[256];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int m = 128; m >= 1; m >>= 1) {
+        if (threadIdx.x < m) red[threadIdx.x] += red[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[e] = accumulate ? out[e] + red[0] : red[0];
+}
+
+template <typename T, bool FUSED>
+void launch_rows(const vbh_matrix* m, const ApplyArgs<T>& a, hipStream_t s) {
+    const int bs = m->uniform_bs;
+    const dim3 grid(a.G), block(kThreads);
+    switch (bs) {
+        case 2: hipLaunchKernelGGL((k_apply_rows<T, 2, FUSED>), grid, block, 0, s, a); break;
+        case 4: hipLaunchKernelGGL((k_apply_rows<T, 4, FUSED>), grid, block, 0, s, a); break;
+        case 8: hipLaunchKernelGGL((k_apply_rows<T, 8, FUSED>), grid, block, 0, s, a); break;
+        default: hipLaunchKernelGGL((k_apply_rows<T, 0, FUSED>), grid, block, 0, s, a); break;
+    }
+    VBH_CHECK_HIP(hipGetLastError());
+}
+
+template <typename T>
+ApplyArgs<T> make_args(const vbh_matrix* m, const T* vals, int K, const T* X, int64_t ldx, T* Y, int64_t ldy) {
+    ApplyArgs<T> a{};
+    a.p = m->pattern();
+    a.vals = vals;
+    a.K = K;
+    a.X = X;
+    a.ldx = ldx;
+    a.Y = Y;
+    a.ldy = ldy;
+    a.alpha = 1.0;
+    a.split = m->split;
+    a.G = m->G;
+    return a;
+}
+
+template <typename T>
+void apply_plain(vbh_matrix* m, const T* vals, int K, const void* X, int64_t ldx, void* Y, int64_t ldy, hipStream_t s) {
+    if (m->nbr == 0 || K == 0) return;
+    auto a = make_args<T>(m, vals, K, (const T*)X, ldx, (T*)Y, ldy);
+    launch_rows<T, false>(m, a, s);
+}
+
+template <typename T>
+void apply_adjoint(vbh_matrix* m, const T* vals, int K, const void* X, int64_t ldx, void* Y, int64_t ldy,
+                   hipStream_t s) {
+    if (m->nbc == 0 || K == 0) return;
+    ensure_adjoint_plan(m, s);
+    AdjointArgs<T> a{};
+    a.p = m->pattern();
+    a.vals = vals;
+    a.K = K;
+    a.X = (const T*)X;
+    a.ldx = ldx;
+    a.Y = (T*)Y;
+    a.ldy = ldy;
+    a.t_ptr = m->t_ptr;
+    a.t_row = m->t_row;
+    a.t_slot = m->t_slot;
+    a.split = m->t_split;
+    a.G = m->tG;
+    hipLaunchKernelGGL(k_apply_adjoint<T>, dim3(a.G), dim3(kThreads), 0, s, a);
+    VBH_CHECK_HIP(hipGetLastError());
+}
+
+void reduce_partials(const double* partials, int G, int E, double* out, bool accumulate, hipStream_t s) {
+    hipLaunchKernelGGL(k_reduce_partials, dim3(E), dim3(256), 0, s, partials, G, E, out, accumulate ? 1 : 0);
+    VBH_CHECK_HIP(hipGetLastError());
+}
+
+template <typename T>
+const T* compute_values(vbh_matrix* m, hipStream_t s) {
+    if constexpr (std::is_same_v<T, float> || std::is_same_v<T, c64>) {
+        ensure_vals32(m, s);
+        return (const T*)m->vals32;
+    } else {
+        return (const T*)m->vals;
+    }
+}
+
+// explicit instantiations used by kpm.hip
+#define VBH_INST(T)                                                                                             \
+    template void launch_rows<T, true>(const vbh_matrix*, const ApplyArgs<T>&, hipStream_t);                   \
+    template void launch_rows<T, false>(const vbh_matrix*, const ApplyArgs<T>&, hipStream_t);                  \
+    template ApplyArgs<T> make_args<T>(const vbh_matrix*, const T*, int, const T*, int64_t, T*, int64_t);      \
+    template const T* compute_values<T>(vbh_matrix*, hipStream_t);                                             \
+    template void apply_plain<T>(vbh_matrix*, const T*, int, const void*, int64_t, void*, int64_t, hipStream_t); \
+    template void apply_adjoint<T>(vbh_matrix*, const T*, int, const void*, int64_t, void*, int64_t, hipStream_t);
+VBH_INST(double)
+VBH_INST(float)
+VBH_INST(c128)
+VBH_INST(c64)
+#undef VBH_INST
+
+}  // namespace vbh
+
+using namespace vbh;
+
+extern "C" {
+
+VBH_API int vbh_spmm(vbh_matrix_t m, int K, const void* X, int ldx, void* Y, int ldy, int precision, void* stream) {
+    return guard([&] {
+        VBH_REQUIRE(m, "null matrix");
+        VBH_REQUIRE(K >= 0 && ldx >= K && ldy >= K, "leading dimensions must be >= K");
+        hipStream_t s = (hipStream_t)stream;
+        dispatch_compute(m, precision, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            apply_plain<T>(m, compute_values<T>(m, s), K, X, ldx, Y, ldy, s);
+        });
+    });
+}
+
+VBH_API int vbh_spmv(vbh_matrix_t m, const void* x, void* y, int precision, void* stream) {
+    return vbh_spmm(m, 1, x, 1, y, 1, precision, stream);
+}
+
+VBH_API int vbh_spmm_adjoint(vbh_matrix_t m, int K, const void* X, int ldx, void* Y, int ldy, int precision,
+                             void* stream) {
+    return guard([&] {
+        VBH_REQUIRE(m, "null matrix");
+        VBH_REQUIRE(K >= 0 && ldx >= K && ldy >= K, "leading dimensions must be >= K");
+        hipStream_t s = (hipStream_t)stream;
+        dispatch_compute(m, precision, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            apply_adjoint<T>(m, compute_values<T>(m, s), K, X, ldx, Y, ldy, s);
+        });
+    });
+}
+
+VBH_API int vbh_spmv_adjoint(vbh_matrix_t m, const void* x, void* y, int precision, void* stream) {
+    return vbh_spmm_adjoint(m, 1, x, 1, y, 1, precision, stream);
+}
+
+VBH_API int vbh_spmm_fused(vbh_matrix_t m, int K, const void* X, int ldx, void* Y, int ldy, double alpha,
+                           double beta, double gamma, const void* Z, int ldz, const void* W, int ldw, double* dots,
+                           int precision, void* stream) {
+    return guard([&] {
+        VBH_REQUIRE(m, "null matrix");
+        VBH_REQUIRE(K >= 0 && ldx >= K && ldy >= K, "leading dimensions must be >= K");
+        VBH_REQUIRE(gamma == 0.0 || (Z && ldz >= K), "gamma != 0 needs Z with ldz >= K");
+        VBH_REQUIRE(!W || ldw >= K, "ldw must be >= K");
+        VBH_REQUIRE(!dots || K <= 64, "fused dot products support K <= 64");
+        // the epilogue reads X at the owned rows (column i of row i): needs nbc >= nbr
+        VBH_REQUIRE(m->nbc >= m->nbr, "fused apply needs the owned columns of a square pattern");
+        if (m->nbr == 0 || K == 0) return;
+        hipStream_t s = (hipStream_t)stream;
+        dispatch_compute(m, precision, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            auto a = make_args<T>(m, compute_values<T>(m, s), K, (const T*)X, ldx, (T*)Y, ldy);
+            a.alpha = alpha;
+            a.beta = beta;
+            a.gamma = gamma;
+            a.Z = (const T*)Z;
+            a.ldz = ldz;
+            a.W = (const T*)W;
+            a.ldw = ldw;
+            double* part = nullptr;
+            if (dots) part = (double*)scratch(sizeof(double) * 2 * K * m->G, 2);
+            a.partials = part;
+            launch_rows<T, true>(m, a, s);
+            if (dots) reduce_partials(part, m->G, 2 * K, dots, true, s);
+        });
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,236 @@
+// Shared definitions of the gfx950 variable-block sparse library: scalar
+// types, the device view of a block pattern, the matrix handle, launch and
+// error helpers.  Written for CDNA4 (64-wide wavefronts, 8 XCDs).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/vbcsr_hip.h"
+
+namespace vbh {
+
+// ---------------------------------------------------------------- errors
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_error(const std::string& msg);
+
+#define VBH_CHECK_HIP(expr)                                                                       \
+    do {                                                                                          \
+        hipError_t _e = (expr);                                                                   \
+        if (_e != hipSuccess)                                                                     \
+            throw ::vbh::Error(VBH_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(_e)); \
+    } while (0)
+
+#define VBH_REQUIRE(cond, msg)                                    \
+    do {                                                          \
+        if (!(cond)) throw ::vbh::Error(VBH_ERR_ARG, (msg));      \
+    } while (0)
+
+template <typename F>
+int guard(F&& f) {
+    try {
+        f();
+        return VBH_OK;
+    } catch (const Error& e) {
+        set_error(e.what());
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return VBH_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return VBH_ERR_ARG;
+    }
+}
+
+// ---------------------------------------------------------------- scalars
+template <typename R>
+struct cplx {
+    R re, im;
+};
+using c64 = cplx<float>;
+using c128 = cplx<double>;
+
+template <typename T>
+struct Sc;  // scalar traits
+
+template <>
+struct Sc<double> {
+    using real = double;
+    static constexpr bool is_complex = false;
+    __host__ __device__ static double zero() { return 0.0; }
+    __host__ __device__ static double make(double re, double) { return re; }
+    __host__ __device__ static double fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+    __host__ __device__ static double conj(double a) { return a; }
+    __host__ __device__ static double re(double a) { return a; }
+    __host__ __device__ static double im(double) { return 0.0; }
+};
+template <>
+struct Sc<float> {
+    using real = float;
+    static constexpr bool is_complex = false;
+    __host__ __device__ static float zero() { return 0.0f; }
+    __host__ __device__ static float make(double re, double) { return float(re); }
+    __host__ __device__ static float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+    __host__ __device__ static float conj(float a) { return a; }
+    __host__ __device__ static double re(float a) { return a; }
+    __host__ __device__ static double im(float) { return 0.0; }
+};
+template <typename R>
+struct Sc<cplx<R>> {
+    using T = cplx<R>;
+    using real = R;
+    static constexpr bool is_complex = true;
+    __host__ __device__ static T zero() { return T{R(0), R(0)}; }
+    __host__ __device__ static T make(double re, double im) { return T{R(re), R(im)}; }
+    // c + a*b
+    __host__ __device__ static T fma(T a, T b, T c) {
+        return T{c.re + a.re * b.re - a.im * b.im, c.im + a.re * b.im + a.im * b.re};
+    }
+    __host__ __device__ static T conj(T a) { return T{a.re, -a.im}; }
+    __host__ __device__ static double re(T a) { return a.re; }
+    __host__ __device__ static double im(T a) { return a.im; }
+};
+
+template <typename R>
+__host__ __device__ inline cplx<R> operator+(cplx<R> a, cplx<R> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename R>
+__host__ __device__ inline cplx<R> operator-(cplx<R> a, cplx<R> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename R>
+__host__ __device__ inline cplx<R> operator*(cplx<R> a, cplx<R> b) {
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename R>
+__host__ __device__ inline cplx<R> operator*(R s, cplx<R> b) { return {s * b.re, s * b.im}; }
+
+// real scale of any scalar
+__host__ __device__ inline double rscale(double s, double a) { return s * a; }
+__host__ __device__ inline float rscale(double s, float a) { return float(s) * a; }
+template <typename R>
+__host__ __device__ inline cplx<R> rscale(double s, cplx<R> a) { return {R(s) * a.re, R(s) * a.im}; }
+// Re(conj(a) * b) in double
+__host__ __device__ inline double redot(double a, double b) { return a * b; }
+__host__ __device__ inline double redot(float a, float b) { return double(a) * double(b); }
+template <typename R>
+__host__ __device__ inline double redot(cplx<R> a, cplx<R> b) {
+    return double(a.re) * double(b.re) + double(a.im) * double(b.im);
+}
+__host__ __device__ inline double sqabs(double a) { return a * a; }
+__host__ __device__ inline double sqabs(float a) { return double(a) * a; }
+template <typename R>
+__host__ __device__ inline double sqabs(cplx<R> a) { return double(a.re) * a.re + double(a.im) * a.im; }
+
+// Single-precision partner of a storage type.
+template <typename T>
+struct Low;
+template <>
+struct Low<double> { using type = float; };
+template <>
+struct Low<c128> { using type = c64; };
+
+// ---------------------------------------------------------------- wave ops
+__device__ inline double shfl_xor(double v, int m) { return __shfl_xor(v, m, 64); }
+__device__ inline float shfl_xor(float v, int m) { return __shfl_xor(v, m, 64); }
+template <typename R>
+__device__ inline cplx<R> shfl_xor(cplx<R> v, int m) {
+    return {__shfl_xor(v.re, m, 64), __shfl_xor(v.im, m, 64)};
+}
+
+__device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__host__ __device__ inline int pow2ceil(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+// Work-group id remapped so that consecutive logical chunks land on the same
+// XCD: the dispatcher hands work-groups to the 8 XCDs round-robin, and each
+// XCD has its own L2, so a contiguous run of block rows (and the x rows they
+// gather) stays in one L2.  Requires G % 8 == 0.
+__device__ inline int xcd_chunk(int G) {
+    const int b = blockIdx.x;
+    const int per = G >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+// ---------------------------------------------------------------- pattern
+// Device view of one block pattern.  Values of block k start at voff[k];
+// element (a, b) of an r x c block is at vals[voff[k] + a*c + b].
+struct DevPattern {
+    int nbr, nbc;
+    int bs;  // uniform block size (0 when variable)
+    const int* rbs;
+    const int* cbs;
+    const int* row_ptr;
+    const int* col_ind;
+    const int* roff;  // nbr+1 scalar row offsets
+    const int* coff;  // nbc+1 scalar column offsets
+    const int64_t* voff;  // nnzb+1
+};
+
+constexpr int kWaves = 4;  // waves per work-group in the row kernels
+constexpr int kThreads = 64 * kWaves;
+
+}  // namespace vbh
+
+// ---------------------------------------------------------------- handle
+struct vbh_matrix {
+    int nbr = 0, nbc = 0;
+    int64_t nnzb = 0, nnz = 0, n_rows = 0, n_cols = 0;
+    int dtype = VBH_F64, kind = 2, max_block = 0, n_classes = 0, uniform_bs = 0;
+    // host pattern
+    std::vector<int> h_rbs, h_cbs, h_row_ptr, h_col_ind;
+    std::vector<int64_t> h_voff;
+    // device pattern
+    int *rbs = nullptr, *cbs = nullptr, *row_ptr = nullptr, *col_ind = nullptr;
+    int *roff = nullptr, *coff = nullptr;
+    int64_t* voff = nullptr;
+    // values
+    void* vals = nullptr;
+    void* vals32 = nullptr;  // FP32 copy, built on demand
+    bool vals32_valid = false;
+    // apply plan: work-balanced chunks of block rows, G chunks (G % 8 == 0)
+    int* split = nullptr;
+    int G = 0;
+    // adjoint plan (CSC of the pattern), built on demand
+    int *t_ptr = nullptr, *t_row = nullptr, *t_slot = nullptr, *t_split = nullptr;
+    int tG = 0;
+    bool t_valid = false;
+
+    vbh::DevPattern pattern() const {
+        return vbh::DevPattern{nbr, nbc, uniform_bs, rbs, cbs, row_ptr, col_ind, roff, coff, voff};
+    }
+    size_t elem_size() const { return dtype == VBH_C128 ? 16 : 8; }
+};
+
+namespace vbh {
+size_t dtype_size(int dtype);
+void* dalloc(size_t bytes);
+void dfree(void* p);
+template <typename T>
+T* upload(const std::vector<T>& h, hipStream_t s = nullptr) {
+    T* d = static_cast<T*>(dalloc(sizeof(T) * (h.empty() ? 1 : h.size())));
+    if (!h.empty()) VBH_CHECK_HIP(hipMemcpyAsync(d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, s));
+    return d;
+}
+// Ensure the FP32 copy of the values exists and is current.
+void ensure_vals32(vbh_matrix* m, hipStream_t s);
+// Ensure the adjoint (column-major incoming) plan exists.
+void ensure_adjoint_plan(vbh_matrix* m, hipStream_t s);
+// Build a matrix handle from host pattern arrays (no value upload).
+vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const int* row_ptr,
+                         const int* col_ind, int dtype, hipStream_t s);
+void destroy_matrix(vbh_matrix* m);
+// Persistent device scratch owned by the library (grown on demand).
+void* scratch(size_t bytes, int slot);
+}  // namespace vbh

@@ -1,0 +1,492 @@
+// KPM moments and Chebyshev series on gfx950.
+//
+// The reference composes the Chebyshev recursion in Python from mult_dense,
+// scale/axpy and bdot (examples/ex08_graphene_dos.py:95-114,
+// examples/inp_nanocrystal_dos/kpm.py:46-73, vbcsr/utils.py:71-126): per
+// moment one SpMM plus ~5 full-vector passes.  Here one fused kernel per
+// recursion step reads A, X = t_n, Z = t_{n-1} once, writes
+// t_{n+1} = 2 H~ t_n - t_{n-1}, and emits per-work-group FP64 partial dots;
+// the partials of all steps are reduced once at the end.  Modes:
+//   0  plain: separate SpMM, axpby and dot kernels, one moment per matvec
+//      (the unfused baseline the fused path is measured against);
+//   1  fused, one moment per matvec: mu_n = <v0, t_n>;
+//   2  fused, two moments per matvec: mu_2n = 2<t_n,t_n> - mu_0,
+//      mu_2n-1 = 2<t_n,t_n-1> - mu_1.
+// precision 1 keeps the recursion vectors and the matrix values in FP32 (the
+// dot products stay FP64).  The batched entry point runs many independent
+// matrices in one launch per step and captures all steps in one HIP graph.
+#include <algorithm>
+#include <cstring>
+
+#include "apply.hpp"
+
+namespace vbh {
+
+template <typename S, typename T>
+__global__ void k_copy_convert(int64_t rows, int K, const S* __restrict__ in, int64_t ldi, T* __restrict__ out,
+                               int64_t ldo) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= rows * K) return;
+    const int64_t r = e / K;
+    const int v = int(e - r * K);
+    const S x = in[r * ldi + v];
+    if constexpr (Sc<S>::is_complex) out[r * ldo + v] = Sc<T>::make(x.re, x.im);
+    else out[r * ldo + v] = Sc<T>::make(double(x), 0.0);
+}
+
+// y = alpha*y + beta*x + gamma*z (the plain path's separate vector pass)
+template <typename T>
+__global__ void k_axpbypcz(int64_t n, double alpha, T* __restrict__ y, double beta, const T* __restrict__ x,
+                           double gamma, const T* __restrict__ z) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    T v = rscale(alpha, y[e]);
+    if (beta != 0.0) v = v + rscale(beta, x[e]);
+    if (gamma != 0.0) v = v + rscale(gamma, z[e]);
+    y[e] = v;
+}
+
+// partial[b][v] = Re sum_rows conj(w) y over rows [b*rp, (b+1)*rp) (plain path bdot)
+template <typename T>
+__global__ __launch_bounds__(256) void k_redot_partials(int64_t rows, int K, const T* __restrict__ w,
+                                                        const T* __restrict__ y, int64_t rp,
+                                                        double* __restrict__ partial, int E, int col0) {
+    __shared__ double red[256];
+    const int t = threadIdx.x;
+    const int Kp = pow2ceil(K);
+    const int v = t & (Kp - 1);
+    const int64_t rb = int64_t(blockIdx.x) * rp;
+    int64_t re_ = rb + rp;
+    if (re_ > rows) re_ = rows;
+    double s = 0.0;
+    if (v < K)
+        for (int64_t r = rb + t / Kp; r < re_; r += 256 / Kp) s += redot(w[r * K + v], y[r * K + v]);
+    red[t] = s;
+    __syncthreads();
+    for (int m = 128; m >= Kp; m >>= 1) {
+        if (t < m) red[t] += red[t + m];
+        __syncthreads();
+    }
+    if (t < K) partial[int64_t(blockIdx.x) * E + col0 + t] = red[t];
+}
+
+// Sum partials per (step, segment): out[(s*nseg + h)*E + e] =
+// sum_{g in seg h} partials[s][g][e]; grid (E, nseg, nsteps).
+__global__ void k_reduce_segments(const double* __restrict__ partials, int Gtot, int E, const int* __restrict__ seg,
+                                  int nseg, double* __restrict__ out) {
+    const int e = blockIdx.x, h = blockIdx.y, s = blockIdx.z;
+    const double* base = partials + int64_t(s) * Gtot * E;
+    double acc = 0.0;
+    for (int g = seg[h] + threadIdx.x; g < seg[h + 1]; g += blockDim.x) acc += base[int64_t(g) * E + e];
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int m = 128; m >= 1; m >>= 1) {
+        if (threadIdx.x < m) red[threadIdx.x] += red[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[(int64_t(s) * nseg + h) * E + e] = red[0];
+}
+
+// ------------------------------------------------------------ batched step
+template <typename T>
+struct BatchItem {
+    DevPattern p;
+    const T* vals;
+    T* buf[3];     // recursion vectors t_{n-1}, t_n, t_{n+1} rotate through these
+    const T* v0;   // mode 1: the start vectors (W of the dot)
+    const int* split;
+    int G;         // this matrix's work-groups
+    int g_base;    // first global work-group
+};
+
+// One launch per recursion step over every matrix of the batch: work-group g
+// (XCD-remapped) finds its matrix by binary search over g_base, then runs the
+// shared row loop on its chunk.  step >= 0 produces t_{step+1}.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_apply_batched(const BatchItem<T>* __restrict__ items, int nitems,
+                                                            int Gtot, int K, int step, double alpha, double beta,
+                                                            double gamma0, double gamma, int mode,
+                                                            double* __restrict__ partials) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int gg = xcd_chunk(Gtot);
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].g_base <= gg) lo = mid;
+        else hi = mid - 1;
+    }
+    const BatchItem<T>& it = items[lo];
+    const int chunk = gg - it.g_base;
+    ApplyArgs<T> a{};
+    a.p = it.p;
+    a.vals = it.vals;
+    a.K = K;
+    a.X = it.buf[step % 3];
+    a.ldx = K;
+    a.Y = it.buf[(step + 1) % 3];
+    a.ldy = K;
+    a.alpha = alpha;
+    a.beta = beta;
+    a.gamma = step == 0 ? gamma0 : gamma;
+    a.Z = it.buf[(step + 2) % 3];
+    a.ldz = K;
+    a.W = mode == 1 ? it.v0 : nullptr;
+    a.ldw = K;
+    double d1 = 0.0, d2 = 0.0;
+    row_loop<T, 0, true>(a, it.split[chunk], it.split[chunk + 1], lane, wave, d1, d2);
+    write_partials(K, lane, wave, d1, d2, partials + int64_t(gg) * 2 * K);
+}
+
+template <typename T>
+static int64_t step_count(int M, int mode) {
+    if (M <= 1) return 0;
+    if (mode == 2) return M / 2;  // steps producing t_1 .. t_ceil((M-1)/2)
+    return M - 1;
+}
+
+// Assemble moments from the per-step column sums d[s][0..K) (d1) and [K..2K) (d2).
+static void assemble_moments(int M, int mode, int K, double mu0, const double* d, double* mu) {
+    for (int n = 0; n < M; ++n) mu[n] = 0.0;
+    mu[0] = mu0;
+    if (M == 1) return;
+    auto sum = [&](int s, int which) {
+        double t = 0.0;
+        for (int v = 0; v < K; ++v) t += d[int64_t(s) * 2 * K + which * K + v];
+        return t;
+    };
+    if (mode != 2) {
+        for (int n = 1; n < M; ++n) mu[n] = sum(n - 1, 0);  // step n-1 produced t_n; d1 = <v0, t_n>
+        return;
+    }
+    // step 0 produced t_1: d1 = <t1,t1>, d2 = <t0,t1>
+    mu[1] = sum(0, 1);
+    for (int s = 0; s < step_count<double>(M, 2); ++s) {
+        const int n = s + 1;  // t_n produced at step s
+        const int even = 2 * n, odd = 2 * n - 1;
+        if (even < M) mu[even] = 2.0 * sum(s, 0) - mu[0];
+        if (odd > 1 && odd < M) mu[odd] = 2.0 * sum(s, 1) - mu[1];
+    }
+}
+
+template <typename T, typename S>
+static void kpm_single(vbh_matrix* m, int K, const S* V0, int ldv, int M, int mode, double center, double half,
+                       double* moments, hipStream_t s) {
+    const int64_t N = m->n_rows;
+    const int64_t nsteps = step_count<T>(M, mode);
+    const size_t vec_bytes = sizeof(T) * std::max<int64_t>(N * K, 1);
+    const bool keep_v0 = mode != 2;  // modes 0/1 dot every t_n against v0
+    T* buf = (T*)scratch(3 * vec_bytes + (keep_v0 ? vec_bytes : 0), 4);
+    T* t[3] = {buf, (T*)((char*)buf + vec_bytes), (T*)((char*)buf + 2 * vec_bytes)};
+    T* v0 = keep_v0 ? (T*)((char*)buf + 3 * vec_bytes) : t[0];
+    const unsigned nb = unsigned((N * K + 255) / 256);
+    if (N * K > 0) {
+        hipLaunchKernelGGL((k_copy_convert<S, T>), dim3(nb), dim3(256), 0, s, N, K, V0, int64_t(ldv), t[0], int64_t(K));
+        VBH_CHECK_HIP(hipGetLastError());
+        if (keep_v0) VBH_CHECK_HIP(hipMemcpyAsync(v0, t[0], sizeof(T) * N * K, hipMemcpyDeviceToDevice, s));
+    }
+    // mu_0 = sum_v <t0, t0>
+    std::vector<double> dd(2 * K);
+    {
+        const int dt = std::is_same_v<T, double> ? VBH_F64 : std::is_same_v<T, float> ? VBH_F32
+                       : std::is_same_v<T, c128> ? VBH_C128 : VBH_C64;
+        if (vbh_vec_bdot(dt, N, K, t[0], K, t[0], K, dd.data(), s) != VBH_OK) throw Error(VBH_ERR_HIP, vbh_last_error());
+    }
+    double mu0 = 0.0;
+    for (int v = 0; v < K; ++v) mu0 += dd[2 * v];
+    if (M <= 1 || N == 0) {
+        for (int n = 0; n < M; ++n) moments[n] = 0.0;
+        if (M >= 1) moments[0] = mu0;
+        return;
+    }
+    const T* vals = compute_values<T>(m, s);
+    const int G = m->G;
+    const int E = 2 * K;
+    double* partials = (double*)scratch(sizeof(double) * nsteps * G * E, 5);
+    const double a1 = 1.0 / half, b1 = -center / half;  // t1 = H~ t0
+    const double a2 = 2.0 / half, b2 = -2.0 * center / half;  // t_{n+1} = 2 H~ t_n - t_{n-1}
+    if (mode == 0) {
+        // plain: SpMM kernel, then a vector pass, then a dot kernel per step
+        const int64_t rp = std::max<int64_t>(256, (N + G - 1) / G);
+        const int ndot = int((N + rp - 1) / rp);
+        VBH_REQUIRE(ndot <= G, "internal: dot partial blocks exceed work-groups");
+        VBH_CHECK_HIP(hipMemsetAsync(partials, 0, sizeof(double) * nsteps * G * E, s));
+        for (int64_t st = 0; st < nsteps; ++st) {
+            T* X = t[st % 3];
+            T* Y = t[(st + 1) % 3];
+            T* Z = t[(st + 2) % 3];
+            apply_plain<T>(m, vals, K, X, K, Y, K, s);
+            hipLaunchKernelGGL(k_axpbypcz<T>, dim3(nb), dim3(256), 0, s, N * K, st == 0 ? a1 : a2, Y,
+                               st == 0 ? b1 : b2, (const T*)X, st == 0 ? 0.0 : -1.0, (const T*)Z);
+            VBH_CHECK_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_redot_partials<T>, dim3(ndot), dim3(256), 0, s, N, K, (const T*)v0,
+                               (const T*)Y, rp, partials + st * G * E, E, 0);
+            VBH_CHECK_HIP(hipGetLastError());
+        }
+    } else {
+        for (int64_t st = 0; st < nsteps; ++st) {
+            auto a = make_args<T>(m, vals, K, t[st % 3], K, t[(st + 1) % 3], K);
+            a.alpha = st == 0 ? a1 : a2;
+            a.beta = st == 0 ? b1 : b2;
+            a.gamma = st == 0 ? 0.0 : -1.0;
+            a.Z = t[(st + 2) % 3];
+            a.ldz = K;
+            a.W = mode == 1 ? v0 : nullptr;
+            a.ldw = K;
+            a.partials = partials + st * G * E;
+            launch_rows<T, true>(m, a, s);
+        }
+    }
+    // one reduction over all steps
+    double* red = (double*)scratch(sizeof(double) * nsteps * E, 6);
+    const int hseg[2] = {0, G};
+    int* dseg = (int*)scratch(sizeof(int) * 2, 7);
+    VBH_CHECK_HIP(hipMemcpyAsync(dseg, hseg, sizeof(int) * 2, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_reduce_segments, dim3(E, 1, unsigned(nsteps)), dim3(256), 0, s, partials, G, E, dseg, 1, red);
+    VBH_CHECK_HIP(hipGetLastError());
+    std::vector<double> d(nsteps * E);
+    VBH_CHECK_HIP(hipMemcpyAsync(d.data(), red, sizeof(double) * nsteps * E, hipMemcpyDeviceToHost, s));
+    VBH_CHECK_HIP(hipStreamSynchronize(s));
+    assemble_moments(M, mode, K, mu0, d.data(), moments);
+}
+
+template <typename F>
+static void dispatch_kpm(const vbh_matrix* m, int precision, F&& f) {
+    VBH_REQUIRE(precision == 0 || precision == 1, "precision must be 0 or 1");
+    if (m->dtype == VBH_F64) {
+        if (precision == 0) f((double*)nullptr, (double*)nullptr);
+        else f((float*)nullptr, (double*)nullptr);
+    } else {
+        if (precision == 0) f((c128*)nullptr, (c128*)nullptr);
+        else f((c64*)nullptr, (c128*)nullptr);
+    }
+}
+
+// ------------------------------------------------------------ batched driver
+template <typename T, typename S>
+static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* const* V0s, const int* ldv, int M, int mode,
+                        double center, double half, int use_graph, double* moments, hipStream_t s) {
+    VBH_REQUIRE(mode == 1 || mode == 2, "batched KPM supports the fused modes 1 and 2");
+    VBH_REQUIRE(K >= 1 && K <= 64, "batched KPM supports 1 <= K <= 64");
+    const int64_t nsteps = step_count<T>(M, mode);
+    std::vector<BatchItem<T>> items(count);
+    std::vector<int> seg(count + 1, 0);
+    std::vector<size_t> off(count + 1, 0);
+    for (int h = 0; h < count; ++h) {
+        VBH_REQUIRE(ms[h]->nbc == ms[h]->nbr, "batched KPM needs square local patterns without ghosts");
+        off[h + 1] = off[h] + sizeof(T) * size_t(ms[h]->n_rows) * K * (mode == 1 ? 4 : 3);
+        seg[h + 1] = seg[h] + ms[h]->G;
+    }
+    const int Gtot = seg[count];
+    const int E = 2 * K;
+    char* vecs = (char*)scratch(std::max<size_t>(off[count], 64), 4);
+    double* partials = (double*)scratch(sizeof(double) * std::max<int64_t>(nsteps, 1) * Gtot * E, 5);
+    std::vector<double> mu0(count, 0.0);
+    for (int h = 0; h < count; ++h) {
+        vbh_matrix* m = ms[h];
+        const int64_t N = m->n_rows;
+        const size_t vb = sizeof(T) * size_t(N) * K;
+        BatchItem<T>& it = items[h];
+        it.p = m->pattern();
+        it.p.bs = 0;
+        it.vals = compute_values<T>(m, s);
+        for (int q = 0; q < 3; ++q) it.buf[q] = (T*)(vecs + off[h] + q * vb);
+        it.v0 = mode == 1 ? (const T*)(vecs + off[h] + 3 * vb) : nullptr;
+        it.split = m->split;
+        it.G = m->G;
+        it.g_base = seg[h];
+        if (N * K > 0) {
+            hipLaunchKernelGGL((k_copy_convert<S, T>), dim3(unsigned((N * K + 255) / 256)), dim3(256), 0, s, N, K,
+                               (const S*)V0s[h], int64_t(ldv[h]), it.buf[0], int64_t(K));
+            VBH_CHECK_HIP(hipGetLastError());
+            if (mode == 1)
+                VBH_CHECK_HIP(hipMemcpyAsync((void*)it.v0, it.buf[0], vb, hipMemcpyDeviceToDevice, s));
+            std::vector<double> dd(2 * K);
+            const int dt = std::is_same_v<T, double> ? VBH_F64 : std::is_same_v<T, float> ? VBH_F32
+                           : std::is_same_v<T, c128> ? VBH_C128 : VBH_C64;
+            if (vbh_vec_bdot(dt, N, K, it.buf[0], K, it.buf[0], K, dd.data(), s) != VBH_OK)
+                throw Error(VBH_ERR_HIP, vbh_last_error());
+            for (int v = 0; v < K; ++v) mu0[h] += dd[2 * v];
+        }
+    }
+    if (nsteps > 0 && Gtot > 0) {
+        BatchItem<T>* ditems = (BatchItem<T>*)scratch(sizeof(BatchItem<T>) * count + sizeof(int) * (count + 1), 6);
+        int* dseg = (int*)(ditems + count);
+        VBH_CHECK_HIP(hipMemcpyAsync(ditems, items.data(), sizeof(BatchItem<T>) * count, hipMemcpyHostToDevice, s));
+        VBH_CHECK_HIP(hipMemcpyAsync(dseg, seg.data(), sizeof(int) * (count + 1), hipMemcpyHostToDevice, s));
+        VBH_CHECK_HIP(hipStreamSynchronize(s));
+        const double a1 = 1.0 / half, b1 = -center / half;
+        const double a2 = 2.0 / half, b2 = -2.0 * center / half;
+        auto launch_all = [&](hipStream_t st) {
+            for (int64_t step = 0; step < nsteps; ++step) {
+                hipLaunchKernelGGL(k_apply_batched<T>, dim3(Gtot), dim3(kThreads), 0, st, (const BatchItem<T>*)ditems,
+                                   count, Gtot, K, int(step), step == 0 ? a1 : a2, step == 0 ? b1 : b2, 0.0, -1.0, mode,
+                                   partials + step * Gtot * E);
+            }
+        };
+        if (use_graph) {
+            hipStream_t cs;
+            VBH_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+            hipGraph_t graph;
+            hipGraphExec_t exec;
+            VBH_CHECK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            launch_all(cs);
+            VBH_CHECK_HIP(hipStreamEndCapture(cs, &graph));
+            VBH_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+            VBH_CHECK_HIP(hipGraphLaunch(exec, cs));
+            VBH_CHECK_HIP(hipStreamSynchronize(cs));
+            VBH_CHECK_HIP(hipGraphExecDestroy(exec));
+            VBH_CHECK_HIP(hipGraphDestroy(graph));
+            VBH_CHECK_HIP(hipStreamDestroy(cs));
+        } else {
+            launch_all(s);
+            VBH_CHECK_HIP(hipGetLastError());
+        }
+        double* red = (double*)scratch(sizeof(double) * nsteps * count * E, 7);
+        hipLaunchKernelGGL(k_reduce_segments, dim3(E, count, unsigned(nsteps)), dim3(256), 0, s, partials, Gtot, E,
+                           dseg, count, red);
+        VBH_CHECK_HIP(hipGetLastError());
+        std::vector<double> d(nsteps * count * E);
+        VBH_CHECK_HIP(hipMemcpyAsync(d.data(), red, sizeof(double) * d.size(), hipMemcpyDeviceToHost, s));
+        VBH_CHECK_HIP(hipStreamSynchronize(s));
+        std::vector<double> dh(nsteps * E);
+        for (int h = 0; h < count; ++h) {
+            for (int64_t st = 0; st < nsteps; ++st)
+                std::memcpy(&dh[st * E], &d[(st * count + h) * E], sizeof(double) * E);
+            assemble_moments(M, mode, K, mu0[h], dh.data(), moments + int64_t(h) * M);
+        }
+    } else {
+        for (int h = 0; h < count; ++h) {
+            for (int n = 0; n < M; ++n) moments[int64_t(h) * M + n] = 0.0;
+            if (M >= 1) moments[int64_t(h) * M] = mu0[h];
+        }
+    }
+}
+
+// ------------------------------------------------------------ Chebyshev apply
+template <typename T>
+__global__ void k_cheb_accum(int64_t rows, int K, const T* __restrict__ t, T* __restrict__ Y, int64_t ldy, double cre,
+                             double cim, int first) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= rows * K) return;
+    const int64_t r = e / K;
+    const int v = int(e - r * K);
+    const T c = Sc<T>::make(cre, cim);
+    const T x = t[r * K + v];
+    T& y = Y[r * ldy + v];
+    y = first ? c * x : Sc<T>::fma(c, x, y);
+}
+__global__ void k_cheb_accum_d(int64_t rows, int K, const double* __restrict__ t, double* __restrict__ Y, int64_t ldy,
+                               double c, int first) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= rows * K) return;
+    const int64_t r = e / K;
+    const int v = int(e - r * K);
+    Y[r * ldy + v] = first ? c * t[r * K + v] : __builtin_fma(c, t[r * K + v], Y[r * ldy + v]);
+}
+__global__ void k_cheb_accum_f(int64_t rows, int K, const float* __restrict__ t, float* __restrict__ Y, int64_t ldy,
+                               double c, int first) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= rows * K) return;
+    const int64_t r = e / K;
+    const int v = int(e - r * K);
+    Y[r * ldy + v] = first ? float(c) * t[r * K + v] : __builtin_fmaf(float(c), t[r * K + v], Y[r * ldy + v]);
+}
+
+template <typename T>
+static void cheb_accum(int64_t N, int K, const T* t, T* Y, int64_t ldy, const double* coef, int n, hipStream_t s) {
+    const unsigned nb = unsigned((N * K + 255) / 256);
+    if constexpr (std::is_same_v<T, double>)
+        hipLaunchKernelGGL(k_cheb_accum_d, dim3(nb), dim3(256), 0, s, N, K, t, Y, ldy, coef[2 * n], n == 0);
+    else if constexpr (std::is_same_v<T, float>)
+        hipLaunchKernelGGL(k_cheb_accum_f, dim3(nb), dim3(256), 0, s, N, K, t, Y, ldy, coef[2 * n], n == 0);
+    else
+        hipLaunchKernelGGL(k_cheb_accum<T>, dim3(nb), dim3(256), 0, s, N, K, t, Y, ldy, coef[2 * n], coef[2 * n + 1],
+                           n == 0);
+    VBH_CHECK_HIP(hipGetLastError());
+}
+
+template <typename T>
+static void cheb_apply(vbh_matrix* m, int K, const T* X, int ldx, T* Y, int ldy, int nterms, const double* coef,
+                       double center, double half, hipStream_t s) {
+    const int64_t N = m->n_rows;
+    if (N * K == 0 || nterms <= 0) return;
+    const size_t vb = sizeof(T) * N * K;
+    T* buf = (T*)scratch(3 * vb, 4);
+    T* t[3] = {buf, (T*)((char*)buf + vb), (T*)((char*)buf + 2 * vb)};
+    const unsigned nb = unsigned((N * K + 255) / 256);
+    hipLaunchKernelGGL((k_copy_convert<T, T>), dim3(nb), dim3(256), 0, s, N, K, X, int64_t(ldx), t[0], int64_t(K));
+    VBH_CHECK_HIP(hipGetLastError());
+    cheb_accum<T>(N, K, t[0], Y, ldy, coef, 0, s);
+    const T* vals = compute_values<T>(m, s);
+    for (int n = 1; n < nterms; ++n) {
+        const int st = n - 1;
+        auto a = make_args<T>(m, vals, K, t[st % 3], K, t[(st + 1) % 3], K);
+        a.alpha = st == 0 ? 1.0 / half : 2.0 / half;
+        a.beta = st == 0 ? -center / half : -2.0 * center / half;
+        a.gamma = st == 0 ? 0.0 : -1.0;
+        a.Z = t[(st + 2) % 3];
+        a.ldz = K;
+        launch_rows<T, true>(m, a, s);
+        cheb_accum<T>(N, K, t[(st + 1) % 3], Y, ldy, coef, n, s);
+    }
+}
+
+}  // namespace vbh
+
+using namespace vbh;
+
+extern "C" {
+
+VBH_API int vbh_kpm_moments(vbh_matrix_t m, int K, const void* V0, int ldv, int M, int mode, double center,
+                            double half_width, int precision, double* moments, void* stream) {
+    return guard([&] {
+        VBH_REQUIRE(m && V0 && moments, "null argument");
+        VBH_REQUIRE(m->nbc == m->nbr, "KPM needs a square local pattern without ghosts");
+        VBH_REQUIRE(mode >= 0 && mode <= 2, "mode must be 0, 1 or 2");
+        VBH_REQUIRE(K >= 1 && K <= 64 && ldv >= K, "KPM supports 1 <= K <= 64 columns per call, ldv >= K");
+        VBH_REQUIRE(half_width > 0.0, "half_width must be positive");
+        if (M <= 0) return;
+        dispatch_kpm(m, precision, [&](auto* t, auto* sty) {
+            using T = std::remove_pointer_t<decltype(t)>;
+            using S = std::remove_pointer_t<decltype(sty)>;
+            kpm_single<T, S>(m, K, (const S*)V0, ldv, M, mode, center, half_width, moments, (hipStream_t)stream);
+        });
+    });
+}
+
+VBH_API int vbh_kpm_moments_batched(int count, const vbh_matrix_t* ms, int K, const void* const* V0s, const int* ldv,
+                                    int M, int mode, double center, double half_width, int precision, int graph,
+                                    double* moments, void* stream) {
+    return guard([&] {
+        VBH_REQUIRE(count >= 1 && ms && V0s && ldv && moments, "null argument");
+        VBH_REQUIRE(half_width > 0.0, "half_width must be positive");
+        for (int h = 0; h < count; ++h) {
+            VBH_REQUIRE(ms[h] && V0s[h] && ldv[h] >= K, "bad batch entry");
+            VBH_REQUIRE(ms[h]->dtype == ms[0]->dtype, "batched KPM needs one dtype");
+        }
+        if (M <= 0) return;
+        dispatch_kpm(ms[0], precision, [&](auto* t, auto* sty) {
+            using T = std::remove_pointer_t<decltype(t)>;
+            using S = std::remove_pointer_t<decltype(sty)>;
+            kpm_batched<T, S>(count, ms, K, V0s, ldv, M, mode, center, half_width, graph, moments, (hipStream_t)stream);
+        });
+    });
+}
+
+VBH_API int vbh_cheb_apply(vbh_matrix_t m, int K, const void* X, int ldx, void* Y, int ldy, int nterms,
+                           const double* coef, double center, double half_width, int precision, void* stream) {
+    return guard([&] {
+        VBH_REQUIRE(m && X && Y && coef, "null argument");
+        VBH_REQUIRE(m->nbc == m->nbr, "Chebyshev apply needs a square local pattern without ghosts");
+        VBH_REQUIRE(ldx >= K && ldy >= K && K >= 1, "bad leading dimensions");
+        VBH_REQUIRE(half_width > 0.0, "half_width must be positive");
+        dispatch_compute(m, precision, [&](auto* tag) {
+            using T = std::remove_pointer_t<decltype(tag)>;
+            cheb_apply<T>(m, K, (const T*)X, ldx, (T*)Y, ldy, nterms, coef, center, half_width, (hipStream_t)stream);
+        });
+    });
+}
+
+}  // extern "C"

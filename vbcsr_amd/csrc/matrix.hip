@@ -1,0 +1,668 @@
+// Matrix handles: construction of the device pattern, value transfer, the
+// FP32 value copy, the adjoint plan, and the value-level matrix algebra
+// (scale, shift, block norms, filter, union axpby, trace).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <numeric>
+
+#include "common.hpp"
+
+namespace vbh {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+size_t dtype_size(int dtype) {
+    switch (dtype) {
+        case VBH_F64: return 8;
+        case VBH_C128: return 16;
+        case VBH_F32: return 4;
+        case VBH_C64: return 8;
+    }
+    throw Error(VBH_ERR_ARG, "unknown dtype");
+}
+
+void* dalloc(size_t bytes) {
+    void* p = nullptr;
+    if (bytes == 0) bytes = 16;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(VBH_ERR_NOMEM, "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
+    }
+    return p;
+}
+
+void dfree(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+// Library scratch: a few grow-only device buffers reused across calls on the
+// current device.  Calls that share a slot must not overlap on one device.
+static std::mutex g_scratch_mu;
+static std::vector<std::pair<void*, size_t>> g_scratch(8, {nullptr, 0});
+void* scratch(size_t bytes, int slot) {
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    auto& s = g_scratch.at(slot);
+    if (s.second < bytes) {
+        if (s.first) {
+            VBH_CHECK_HIP(hipDeviceSynchronize());
+            dfree(s.first);
+        }
+        s.first = dalloc(bytes);
+        s.second = bytes;
+    }
+    return s.first;
+}
+
+// ---------------------------------------------------------------- kernels
+template <typename T, typename L>
+__global__ void k_convert(const T* __restrict__ in, L* __restrict__ out, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if constexpr (Sc<T>::is_complex) out[i] = L{(float)in[i].re, (float)in[i].im};
+    else out[i] = (L)in[i];
+}
+
+template <typename T>
+__global__ void k_scale(T* __restrict__ v, int64_t n, T alpha) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = v[i] * alpha;
+}
+__global__ void k_scale_d(double* __restrict__ v, int64_t n, double alpha) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) v[i] *= alpha;
+}
+
+// Frobenius norm per block: one wave per block, FP64 accumulation.
+template <typename T>
+__global__ __launch_bounds__(256) void k_block_norms(const T* __restrict__ vals, const int64_t* __restrict__ voff,
+                                                     int64_t nnzb, double* __restrict__ out) {
+    const int64_t k = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (k >= nnzb) return;
+    const int64_t b = voff[k], e = voff[k + 1];
+    double s = 0.0;
+    for (int64_t i = b + lane; i < e; i += 64) s += sqabs(vals[i]);
+    for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+    if (lane == 0) out[k] = sqrt(s);
+}
+
+// shift: alpha added to the leading min(r,c) diagonal of row i's own block.
+template <typename T>
+__global__ void k_shift(T* __restrict__ vals, const int64_t* __restrict__ voff, const int* __restrict__ diag_slot,
+                        const int* __restrict__ rbs, const int* __restrict__ cbs, const int* __restrict__ col_ind,
+                        int nbr, T alpha) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nbr) return;
+    const int k = diag_slot[i];
+    if (k < 0) return;
+    const int r = rbs[i], c = cbs[col_ind[k]];
+    const int d = r < c ? r : c;
+    T* blk = vals + voff[k];
+    for (int t = 0; t < d; ++t) blk[int64_t(t) * c + t] = blk[int64_t(t) * c + t] + alpha;
+}
+
+// Copy selected blocks: dst block q takes src block src_slot[q] (whole block).
+template <typename T>
+__global__ __launch_bounds__(256) void k_gather_blocks(const T* __restrict__ src, const int64_t* __restrict__ src_voff,
+                                                       const int* __restrict__ src_slot, T* __restrict__ dst,
+                                                       const int64_t* __restrict__ dst_voff, int64_t nq) {
+    const int64_t q = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (q >= nq) return;
+    const int64_t sb = src_voff[src_slot[q]];
+    const int64_t db = dst_voff[q], de = dst_voff[q + 1];
+    for (int64_t i = lane; i < de - db; i += 64) dst[db + i] = src[sb + i];
+}
+
+// Union axpby: dst block q = beta*Y[ys[q]] (if ys >= 0) + alpha*X[xs[q]] (if xs >= 0).
+template <typename T>
+__global__ __launch_bounds__(256) void k_union_axpby(const T* __restrict__ Y, const int64_t* __restrict__ yvoff,
+                                                     const int* __restrict__ ys, const T* __restrict__ X,
+                                                     const int64_t* __restrict__ xvoff, const int* __restrict__ xs,
+                                                     T* __restrict__ dst, const int64_t* __restrict__ dvoff,
+                                                     int64_t nq, T alpha, T beta) {
+    const int64_t q = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (q >= nq) return;
+    const int64_t db = dvoff[q], n = dvoff[q + 1] - db;
+    const int yq = ys[q], xq = xs[q];
+    for (int64_t i = lane; i < n; i += 64) {
+        T acc = Sc<T>::zero();
+        if (yq >= 0) acc = beta * Y[yvoff[yq] + i];
+        if (xq >= 0) acc = Sc<T>::fma(alpha, X[xvoff[xq] + i], acc);
+        dst[db + i] = acc;
+    }
+}
+template <typename T>
+__global__ void k_axpby_flat(const T* __restrict__ X, T* __restrict__ Y, int64_t n, T alpha, T beta) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) Y[i] = Sc<T>::fma(alpha, X[i], beta * Y[i]);
+}
+__global__ void k_axpby_flat_d(const double* __restrict__ X, double* __restrict__ Y, int64_t n, double alpha,
+                               double beta) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) Y[i] = __builtin_fma(alpha, X[i], beta * Y[i]);
+}
+
+// Diagonal trace: sum over rows of the diagonal of the row's own block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_trace(const T* __restrict__ vals, const int64_t* __restrict__ voff,
+                                               const int* __restrict__ diag_slot, const int* __restrict__ rbs,
+                                               int nbr, double* __restrict__ partial) {
+    __shared__ double red[256];
+    double s = 0.0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nbr; i += gridDim.x * blockDim.x) {
+        const int k = diag_slot[i];
+        if (k < 0) continue;
+        const int r = rbs[i];
+        for (int t = 0; t < r; ++t) s += Sc<T>::re(vals[voff[k] + int64_t(t) * r + t]);
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int m = 128; m >= 1; m >>= 1) {
+        if (threadIdx.x < m) red[threadIdx.x] += red[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// ---------------------------------------------------------------- helpers
+static inline unsigned blocks_for(int64_t n, int per) { return unsigned((n + per - 1) / per); }
+
+template <typename F>
+static void dispatch_dtype(int dtype, F&& f) {
+    if (dtype == VBH_F64) f(double{});
+    else if (dtype == VBH_C128) f(c128{});
+    else throw Error(VBH_ERR_NOTSUP, "matrix dtype must be VBH_F64 or VBH_C128");
+}
+
+template <typename T>
+static T scalar_from(const double* a) {
+    return Sc<T>::make(a[0], a[1]);
+}
+
+vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const int* row_ptr, const int* col_ind,
+                         int dtype, hipStream_t s) {
+    VBH_REQUIRE(nbr >= 0 && nbc >= 0, "negative block counts");
+    VBH_REQUIRE(dtype == VBH_F64 || dtype == VBH_C128, "matrix dtype must be VBH_F64 or VBH_C128");
+    auto* m = new vbh_matrix();
+    try {
+        m->nbr = nbr;
+        m->nbc = nbc;
+        m->dtype = dtype;
+        m->h_rbs.assign(rbs, rbs + nbr);
+        m->h_cbs.assign(cbs, cbs + nbc);
+        m->h_row_ptr.assign(row_ptr, row_ptr + nbr + 1);
+        VBH_REQUIRE(m->h_row_ptr[0] == 0, "row_ptr must start at 0");
+        for (int i = 0; i < nbr; ++i) VBH_REQUIRE(row_ptr[i + 1] >= row_ptr[i], "row_ptr not monotone");
+        m->nnzb = row_ptr[nbr];
+        m->h_col_ind.assign(col_ind, col_ind + m->nnzb);
+        int mn = 1 << 30, mx = 0;
+        for (int i = 0; i < nbr; ++i) {
+            VBH_REQUIRE(rbs[i] > 0, "block sizes must be positive");
+            mn = std::min(mn, rbs[i]);
+            mx = std::max(mx, rbs[i]);
+        }
+        for (int j = 0; j < nbc; ++j) {
+            VBH_REQUIRE(cbs[j] > 0, "block sizes must be positive");
+            mn = std::min(mn, cbs[j]);
+            mx = std::max(mx, cbs[j]);
+        }
+        m->max_block = mx;
+        // kind detection: block_csr.hpp:2770-2798 (on the owned row sizes)
+        int rmn = 1 << 30, rmx = 0;
+        for (int i = 0; i < nbr; ++i) { rmn = std::min(rmn, rbs[i]); rmx = std::max(rmx, rbs[i]); }
+        if (nbr == 0) { rmn = 1; rmx = 1; }
+        m->kind = rmx <= 1 ? 0 : (rmn == rmx ? 1 : 2);
+        m->uniform_bs = (mn == mx && nbc > 0) ? mx : 0;
+        std::vector<int> roff(nbr + 1, 0), coff(nbc + 1, 0);
+        for (int i = 0; i < nbr; ++i) roff[i + 1] = roff[i] + rbs[i];
+        for (int j = 0; j < nbc; ++j) coff[j + 1] = coff[j] + cbs[j];
+        m->n_rows = roff[nbr];
+        m->n_cols = coff[nbc];
+        m->h_voff.assign(m->nnzb + 1, 0);
+        std::vector<int64_t> work(nbr + 1, 0);
+        std::vector<char> cls(size_t(mx + 1) * (mx + 1), 0);
+        for (int i = 0; i < nbr; ++i) {
+            int64_t w = 64;
+            for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+                const int j = col_ind[k];
+                VBH_REQUIRE(j >= 0 && j < nbc, "column index out of range");
+                if (k > row_ptr[i]) VBH_REQUIRE(col_ind[k - 1] < j, "columns must be strictly ascending per row");
+                const int64_t sz = int64_t(rbs[i]) * cbs[j];
+                m->h_voff[k + 1] = m->h_voff[k] + sz;
+                w += sz + 16;
+                cls[size_t(rbs[i]) * (mx + 1) + cbs[j]] = 1;
+            }
+            work[i + 1] = work[i] + w;
+        }
+        m->n_classes = int(std::count(cls.begin(), cls.end(), 1));
+        m->nnz = m->h_voff[m->nnzb];
+        // work-balanced chunks of rows, G a multiple of 8 (one run per XCD)
+        int G = int(std::min<int64_t>(std::max<int64_t>((nbr + kWaves - 1) / kWaves, 8), 4096));
+        G = (G + 7) & ~7;
+        std::vector<int> split(G + 1, nbr);
+        split[0] = 0;
+        const int64_t total = work[nbr];
+        int row = 0;
+        for (int g = 1; g < G; ++g) {
+            const int64_t target = total * g / G;
+            while (row < nbr && work[row] < target) ++row;
+            split[g] = row;
+        }
+        split[G] = nbr;
+        m->G = G;
+        m->rbs = upload(m->h_rbs, s);
+        m->cbs = upload(m->h_cbs, s);
+        m->row_ptr = upload(m->h_row_ptr, s);
+        m->col_ind = upload(m->h_col_ind, s);
+        m->roff = upload(roff, s);
+        m->coff = upload(coff, s);
+        m->voff = upload(m->h_voff, s);
+        m->split = upload(split, s);
+        m->vals = dalloc(m->elem_size() * std::max<int64_t>(m->nnz, 1));
+        VBH_CHECK_HIP(hipMemsetAsync(m->vals, 0, m->elem_size() * std::max<int64_t>(m->nnz, 1), s));
+        VBH_CHECK_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        destroy_matrix(m);
+        throw;
+    }
+    return m;
+}
+
+void destroy_matrix(vbh_matrix* m) {
+    if (!m) return;
+    for (void* p : {(void*)m->rbs, (void*)m->cbs, (void*)m->row_ptr, (void*)m->col_ind, (void*)m->roff,
+                    (void*)m->coff, (void*)m->voff, m->vals, m->vals32, (void*)m->split, (void*)m->t_ptr,
+                    (void*)m->t_row, (void*)m->t_slot, (void*)m->t_split})
+        dfree(p);
+    delete m;
+}
+
+void ensure_vals32(vbh_matrix* m, hipStream_t s) {
+    if (m->vals32_valid) return;
+    if (!m->vals32) m->vals32 = dalloc((m->elem_size() / 2) * std::max<int64_t>(m->nnz, 1));
+    const int64_t n = m->nnz;
+    if (n > 0) {
+        if (m->dtype == VBH_F64)
+            hipLaunchKernelGGL((k_convert<double, float>), dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                               (const double*)m->vals, (float*)m->vals32, n);
+        else
+            hipLaunchKernelGGL((k_convert<c128, c64>), dim3(blocks_for(n, 256)), dim3(256), 0, s,
+                               (const c128*)m->vals, (c64*)m->vals32, n);
+        VBH_CHECK_HIP(hipGetLastError());
+    }
+    m->vals32_valid = true;
+}
+
+// Column-major "incoming" plan for the adjoint: for every local column j the
+// (row, slot) pairs in ascending row order -- one writer per output block, as
+// in the reference's adjoint plan (vbcsr_backend.hpp:543-625).
+void ensure_adjoint_plan(vbh_matrix* m, hipStream_t s) {
+    if (m->t_valid) return;
+    std::vector<int> tp(m->nbc + 1, 0), tr(m->nnzb), ts(m->nnzb);
+    for (int64_t k = 0; k < m->nnzb; ++k) tp[m->h_col_ind[k] + 1]++;
+    for (int j = 0; j < m->nbc; ++j) tp[j + 1] += tp[j];
+    std::vector<int> fill(tp.begin(), tp.end() - 1);
+    for (int i = 0; i < m->nbr; ++i)
+        for (int k = m->h_row_ptr[i]; k < m->h_row_ptr[i + 1]; ++k) {
+            const int j = m->h_col_ind[k];
+            tr[fill[j]] = i;
+            ts[fill[j]] = k;
+            ++fill[j];
+        }
+    std::vector<int64_t> work(m->nbc + 1, 0);
+    for (int j = 0; j < m->nbc; ++j) {
+        int64_t w = 64;
+        for (int q = tp[j]; q < tp[j + 1]; ++q) w += int64_t(m->h_rbs[tr[q]]) * m->h_cbs[j] + 16;
+        work[j + 1] = work[j] + w;
+    }
+    int G = int(std::min<int64_t>(std::max<int64_t>((m->nbc + kWaves - 1) / kWaves, 8), 4096));
+    G = (G + 7) & ~7;
+    std::vector<int> split(G + 1, m->nbc);
+    split[0] = 0;
+    int col = 0;
+    for (int g = 1; g < G; ++g) {
+        const int64_t target = work[m->nbc] * g / G;
+        while (col < m->nbc && work[col] < target) ++col;
+        split[g] = col;
+    }
+    m->t_ptr = upload(tp, s);
+    m->t_row = upload(tr, s);
+    m->t_slot = upload(ts, s);
+    m->t_split = upload(split, s);
+    m->tG = G;
+    VBH_CHECK_HIP(hipStreamSynchronize(s));
+    m->t_valid = true;
+}
+
+static std::vector<int> diag_slots(const vbh_matrix* m, const int* diag_col) {
+    std::vector<int> d(m->nbr, -1);
+    for (int i = 0; i < m->nbr; ++i) {
+        const int want = diag_col ? diag_col[i] : i;
+        if (want < 0) continue;
+        for (int k = m->h_row_ptr[i]; k < m->h_row_ptr[i + 1]; ++k)
+            if (m->h_col_ind[k] == want) { d[i] = k; break; }
+    }
+    return d;
+}
+
+}  // namespace vbh
+
+using namespace vbh;
+
+// ============================================================== C ABI
+extern "C" {
+
+VBH_API const char* vbh_last_error(void) { return g_last_error.c_str(); }
+VBH_API int vbh_version(void) { return 100; }
+
+VBH_API int vbh_device_count(int* count) {
+    return guard([&] { VBH_CHECK_HIP(hipGetDeviceCount(count)); });
+}
+VBH_API int vbh_set_device(int device) {
+    return guard([&] { VBH_CHECK_HIP(hipSetDevice(device)); });
+}
+VBH_API int vbh_malloc(void** ptr, size_t bytes) {
+    return guard([&] { *ptr = dalloc(bytes); });
+}
+VBH_API int vbh_free(void* ptr) {
+    return guard([&] { dfree(ptr); });
+}
+VBH_API int vbh_memcpy(void* dst, const void* src, size_t bytes, int kind, void* stream) {
+    return guard([&] {
+        if (bytes == 0) return;
+        const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice
+                                : kind == 1 ? hipMemcpyDeviceToHost
+                                            : hipMemcpyDeviceToDevice;
+        VBH_CHECK_HIP(hipMemcpyAsync(dst, src, bytes, k, (hipStream_t)stream));
+        if (kind == 1) VBH_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream));
+    });
+}
+VBH_API int vbh_memset(void* dst, int value, size_t bytes, void* stream) {
+    return guard([&] {
+        if (bytes) VBH_CHECK_HIP(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream));
+    });
+}
+VBH_API int vbh_stream_sync(void* stream) {
+    return guard([&] { VBH_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream)); });
+}
+
+VBH_API int vbh_matrix_create(int nbr, int nbc, const int* rbs, const int* cbs, const int* row_ptr,
+                              const int* col_ind, int dtype, vbh_matrix_t* out) {
+    return guard([&] {
+        VBH_REQUIRE(out, "null output handle");
+        *out = build_matrix(nbr, nbc, rbs, cbs, row_ptr, col_ind, dtype, nullptr);
+    });
+}
+
+VBH_API int vbh_matrix_destroy(vbh_matrix_t m) {
+    return guard([&] { destroy_matrix(m); });
+}
+
+VBH_API int vbh_matrix_info_get(vbh_matrix_t m, vbh_matrix_info* info) {
+    return guard([&] {
+        VBH_REQUIRE(m && info, "null argument");
+        info->nbr = m->nbr;
+        info->nbc = m->nbc;
+        info->nnzb = m->nnzb;
+        info->nnz = m->nnz;
+        info->n_rows = m->n_rows;
+        info->n_cols = m->n_cols;
+        info->dtype = m->dtype;
+        info->kind = m->kind;
+        info->max_block = m->max_block;
+        info->shape_classes = m->n_classes;
+    });
+}
+
+VBH_API int vbh_matrix_get_pattern(vbh_matrix_t m, int* row_ptr, int* col_ind) {
+    return guard([&] {
+        VBH_REQUIRE(m, "null matrix");
+        std::copy(m->h_row_ptr.begin(), m->h_row_ptr.end(), row_ptr);
+        std::copy(m->h_col_ind.begin(), m->h_col_ind.end(), col_ind);
+    });
+}
+
+VBH_API int vbh_matrix_set_values(vbh_matrix_t m, const void* vals, int location, void* stream) {
+    return guard([&] {
+        VBH_REQUIRE(m, "null matrix");
+        const size_t bytes = m->elem_size() * m->nnz;
+        if (bytes)
+            VBH_CHECK_HIP(hipMemcpyAsync(m->vals, vals, bytes,
+                                         location ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                                         (hipStream_t)stream));
+        if (!location) VBH_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream));
+        m->vals32_valid = false;
+    });
+}
+
+VBH_API int vbh_matrix_get_values(vbh_matrix_t m, void* vals, int location, void* stream) {
+    return guard([&] {
+        VBH_REQUIRE(m, "null matrix");
+        const size_t bytes = m->elem_size() * m->nnz;
+        if (bytes)
+            VBH_CHECK_HIP(hipMemcpyAsync(vals, m->vals, bytes,
+                                         location ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                         (hipStream_t)stream));
+        if (!location) VBH_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream));
+    });
+}
+
+VBH_API void* vbh_matrix_values(vbh_matrix_t m) { return m ? m->vals : nullptr; }
+
+VBH_API int vbh_matrix_duplicate(vbh_matrix_t m, vbh_matrix_t* out, void* stream) {
+    return guard([&] {
+        VBH_REQUIRE(m && out, "null argument");
+        hipStream_t s = (hipStream_t)stream;
+        vbh_matrix* c = build_matrix(m->nbr, m->nbc, m->h_rbs.data(), m->h_cbs.data(), m->h_row_ptr.data(),
+                                     m->h_col_ind.data(), m->dtype, s);
+        if (m->nnz)
+            VBH_CHECK_HIP(hipMemcpyAsync(c->vals, m->vals, m->elem_size() * m->nnz, hipMemcpyDeviceToDevice, s));
+        *out = c;
+    });
+}
+
+VBH_API int vbh_matrix_scale(vbh_matrix_t m, const double* alpha, void* stream) {
+    return guard([&] {
+        VBH_REQUIRE(m && alpha, "null argument");
+        hipStream_t s = (hipStream_t)stream;
+        if (m->nnz == 0) return;
+        dispatch_dtype(m->dtype, [&](auto tag) {
+            using T = decltype(tag);
+            if constexpr (std::is_same_v<T, double>)
+                hipLaunchKernelGGL(k_scale_d, dim3(blocks_for(m->nnz, 256)), dim3(256), 0, s, (double*)m->vals,
+                                   m->nnz, alpha[0]);
+            else
+                hipLaunchKernelGGL(k_scale<T>, dim3(blocks_for(m->nnz, 256)), dim3(256), 0, s, (T*)m->vals,
+                                   m->nnz, scalar_from<T>(alpha));
+        });
+        VBH_CHECK_HIP(hipGetLastError());
+        m->vals32_valid = false;
+    });
+}
+
+VBH_API int vbh_matrix_shift(vbh_matrix_t m, const double* alpha, const int* diag_col_host, void* stream) {
+    return guard([&] {
+        VBH_REQUIRE(m && alpha, "null argument");
+        hipStream_t s = (hipStream_t)stream;
+        if (m->nbr == 0) return;
+        auto ds = diag_slots(m, diag_col_host);
+        int* d = upload(ds, s);
+        dispatch_dtype(m->dtype, [&](auto tag) {
+            using T = decltype(tag);
+            hipLaunchKernelGGL(k_shift<T>, dim3(blocks_for(m->nbr, 256)), dim3(256), 0, s, (T*)m->vals, m->voff, d,
+                               m->rbs, m->cbs, m->col_ind, m->nbr, scalar_from<T>(alpha));
+        });
+        VBH_CHECK_HIP(hipGetLastError());
+        VBH_CHECK_HIP(hipStreamSynchronize(s));
+        dfree(d);
+        m->vals32_valid = false;
+    });
+}
+
+VBH_API int vbh_matrix_block_norms(vbh_matrix_t m, double* norms_dev, void* stream) {
+    return guard([&] {
+        VBH_REQUIRE(m && norms_dev, "null argument");
+        if (m->nnzb == 0) return;
+        hipStream_t s = (hipStream_t)stream;
+        dispatch_dtype(m->dtype, [&](auto tag) {
+            using T = decltype(tag);
+            hipLaunchKernelGGL(k_block_norms<T>, dim3(blocks_for(m->nnzb, 4)), dim3(256), 0, s, (const T*)m->vals,
+                               m->voff, m->nnzb, norms_dev);
+        });
+        VBH_CHECK_HIP(hipGetLastError());
+    });
+}
+
+// New matrix with the given pattern whose blocks are gathered from m's slots.
+static vbh_matrix* gather_matrix(vbh_matrix* m, const std::vector<int>& rp, const std::vector<int>& ci,
+                                 const std::vector<int>& src_slot, hipStream_t s) {
+    vbh_matrix* c = build_matrix(m->nbr, m->nbc, m->h_rbs.data(), m->h_cbs.data(), rp.data(), ci.data(), m->dtype, s);
+    if (c->nnzb) {
+        int* d_src = upload(src_slot, s);
+        dispatch_dtype(m->dtype, [&](auto tag) {
+            using T = decltype(tag);
+            hipLaunchKernelGGL(k_gather_blocks<T>, dim3(blocks_for(c->nnzb, 4)), dim3(256), 0, s,
+                               (const T*)m->vals, m->voff, d_src, (T*)c->vals, c->voff, c->nnzb);
+        });
+        VBH_CHECK_HIP(hipGetLastError());
+        VBH_CHECK_HIP(hipStreamSynchronize(s));
+        dfree(d_src);
+    }
+    return c;
+}
+
+VBH_API int vbh_matrix_filter(vbh_matrix_t m, double thr, vbh_matrix_t* out, void* stream) {
+    return guard([&] {
+        VBH_REQUIRE(m && out, "null argument");
+        *out = nullptr;
+        if (!(thr > 0.0) || m->nnzb == 0) return;  // block_csr.hpp:1400
+        hipStream_t s = (hipStream_t)stream;
+        double* d_norms = (double*)scratch(sizeof(double) * m->nnzb, 0);
+        dispatch_dtype(m->dtype, [&](auto tag) {
+            using T = decltype(tag);
+            hipLaunchKernelGGL(k_block_norms<T>, dim3(blocks_for(m->nnzb, 4)), dim3(256), 0, s, (const T*)m->vals,
+                               m->voff, m->nnzb, d_norms);
+        });
+        VBH_CHECK_HIP(hipGetLastError());
+        std::vector<double> norms(m->nnzb);
+        VBH_CHECK_HIP(hipMemcpyAsync(norms.data(), d_norms, sizeof(double) * m->nnzb, hipMemcpyDeviceToHost, s));
+        VBH_CHECK_HIP(hipStreamSynchronize(s));
+        std::vector<int> rp(m->nbr + 1, 0), ci, src;
+        ci.reserve(m->nnzb);
+        src.reserve(m->nnzb);
+        for (int i = 0; i < m->nbr; ++i) {
+            for (int k = m->h_row_ptr[i]; k < m->h_row_ptr[i + 1]; ++k) {
+                if (norms[k] < thr) continue;  // keep >= thr: block_csr.hpp:1411
+                ci.push_back(m->h_col_ind[k]);
+                src.push_back(k);
+            }
+            rp[i + 1] = int(ci.size());
+        }
+        if (int64_t(ci.size()) == m->nnzb) return;
+        *out = gather_matrix(m, rp, ci, src, s);
+    });
+}
+
+VBH_API int vbh_matrix_axpby(vbh_matrix_t self, const double* alpha, vbh_matrix_t X, const double* beta,
+                             vbh_matrix_t* out, void* stream) {
+    return guard([&] {
+        VBH_REQUIRE(self && X && alpha && beta && out, "null argument");
+        VBH_REQUIRE(self->dtype == X->dtype, "axpby: dtype mismatch");
+        VBH_REQUIRE(self->nbr == X->nbr, "Matrix row count mismatch in axpby");
+        *out = nullptr;
+        hipStream_t s = (hipStream_t)stream;
+        const bool same = self->h_row_ptr == X->h_row_ptr && self->h_col_ind == X->h_col_ind &&
+                          self->h_rbs == X->h_rbs && self->h_cbs == X->h_cbs;
+        if (same) {
+            if (self->nnz == 0) return;
+            dispatch_dtype(self->dtype, [&](auto tag) {
+                using T = decltype(tag);
+                if constexpr (std::is_same_v<T, double>)
+                    hipLaunchKernelGGL(k_axpby_flat_d, dim3(blocks_for(self->nnz, 256)), dim3(256), 0, s,
+                                       (const double*)X->vals, (double*)self->vals, self->nnz, alpha[0], beta[0]);
+                else
+                    hipLaunchKernelGGL(k_axpby_flat<T>, dim3(blocks_for(self->nnz, 256)), dim3(256), 0, s,
+                                       (const T*)X->vals, (T*)self->vals, self->nnz, scalar_from<T>(alpha),
+                                       scalar_from<T>(beta));
+            });
+            VBH_CHECK_HIP(hipGetLastError());
+            self->vals32_valid = false;
+            return;
+        }
+        VBH_REQUIRE(self->h_rbs == X->h_rbs, "axpby: block row sizes differ");
+        // union pattern, sorted columns (ops/axpby.hpp:417-527)
+        const int nbc = std::max(self->nbc, X->nbc);
+        std::vector<int> cbs(nbc);
+        for (int j = 0; j < nbc; ++j) {
+            const int a = j < self->nbc ? self->h_cbs[j] : 0;
+            const int b = j < X->nbc ? X->h_cbs[j] : 0;
+            VBH_REQUIRE(a == 0 || b == 0 || a == b, "axpby: column block sizes differ");
+            cbs[j] = a ? a : b;
+        }
+        std::vector<int> rp(self->nbr + 1, 0), ci, ys, xs;
+        for (int i = 0; i < self->nbr; ++i) {
+            int ky = self->h_row_ptr[i], kx = X->h_row_ptr[i];
+            const int ey = self->h_row_ptr[i + 1], ex = X->h_row_ptr[i + 1];
+            while (ky < ey || kx < ex) {
+                const int cy = ky < ey ? self->h_col_ind[ky] : INT32_MAX;
+                const int cx = kx < ex ? X->h_col_ind[kx] : INT32_MAX;
+                const int col = std::min(cy, cx);
+                ci.push_back(col);
+                ys.push_back(cy == col ? ky++ : -1);
+                xs.push_back(cx == col ? kx++ : -1);
+            }
+            rp[i + 1] = int(ci.size());
+        }
+        vbh_matrix* c = build_matrix(self->nbr, nbc, self->h_rbs.data(), cbs.data(), rp.data(), ci.data(),
+                                     self->dtype, s);
+        if (c->nnzb) {
+            int* d_ys = upload(ys, s);
+            int* d_xs = upload(xs, s);
+            dispatch_dtype(self->dtype, [&](auto tag) {
+                using T = decltype(tag);
+                hipLaunchKernelGGL(k_union_axpby<T>, dim3(blocks_for(c->nnzb, 4)), dim3(256), 0, s,
+                                   (const T*)self->vals, self->voff, d_ys, (const T*)X->vals, X->voff, d_xs,
+                                   (T*)c->vals, c->voff, c->nnzb, scalar_from<T>(alpha), scalar_from<T>(beta));
+            });
+            VBH_CHECK_HIP(hipGetLastError());
+            VBH_CHECK_HIP(hipStreamSynchronize(s));
+            dfree(d_ys);
+            dfree(d_xs);
+        }
+        *out = c;
+    });
+}
+
+VBH_API int vbh_matrix_trace(vbh_matrix_t m, double* out, void* stream) {
+    return guard([&] {
+        VBH_REQUIRE(m && out, "null argument");
+        hipStream_t s = (hipStream_t)stream;
+        out[0] = 0.0;
+        if (m->nbr == 0) return;
+        auto ds = diag_slots(m, nullptr);
+        int* d = upload(ds, s);
+        const int nb = 64;
+        double* part = (double*)scratch(sizeof(double) * nb, 1);
+        dispatch_dtype(m->dtype, [&](auto tag) {
+            using T = decltype(tag);
+            hipLaunchKernelGGL(k_trace<T>, dim3(nb), dim3(256), 0, s, (const T*)m->vals, m->voff, d, m->rbs, m->nbr,
+                               part);
+        });
+        VBH_CHECK_HIP(hipGetLastError());
+        std::vector<double> h(nb);
+        VBH_CHECK_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
+        VBH_CHECK_HIP(hipStreamSynchronize(s));
+        dfree(d);
+        double t = 0.0;
+        for (double v : h) t += v;
+        out[0] = t;
+    });
+}
+
+}  // extern "C"

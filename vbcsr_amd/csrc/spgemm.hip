@@ -1,0 +1,547 @@
+// Thresholded block SpGEMM C = A B on gfx950.
+//
+// Rules (the reference's, restated): block norms are Frobenius norms in FP64
+// (block_csr.hpp:2486-2502); the symbolic estimate est_ij = sum_k
+// |A_ik||B_kj| keeps column j iff est_ij > thr, strictly
+// (ops/spmm/common.hpp:507-690, keep rule :664); the numeric phase skips a
+// pair when |A_ik||B_kj| < thr / max(1, nnzb_row_A(i)) (ops/spmm/vbcsr.hpp:
+// 371-387) and accumulates the rest into C's symbolic blocks; finally, when
+// thr > 0, blocks with |C_ij| < thr are dropped (block_csr.hpp:1399-1528).
+//
+// Symbolic phase: one work-group per row of A with an open-addressing hash
+// table (key = global column, value = FP64 estimate) sized to
+// pow2ceil(2 min(bound_i, nbc) + 1), where bound_i = sum_k nnzb(B_k) is the
+// number of multiplication paths of the row.  Tables up to 4096 slots live in
+// LDS (three size classes, so small rows do not pay for big tables); wider
+// rows use a per-row slice of device memory.  Kept columns are written
+// unsorted to a per-row region; the host sorts each row (ascending global
+// column, common.hpp:668) and builds C's pattern.
+//
+// Numeric phase: one work-group per row of C.  The A slots of the row are
+// visited in order; for each, the four waves take the B slots of row k
+// round-robin, locate C_ij by binary search in C's sorted row, and one lane
+// per element of the r x c output block accumulates sum_m A[a][m] B[m][b].
+// A work-group barrier separates consecutive A slots, so every C element is
+// updated by one thread at a time in a fixed order -- deterministic, no
+// atomics.  numeric = 1 runs the block products in FP32 (FP32 accumulation,
+// FP64 result); numeric = 2 rounds A and B to BF16 and multiplies on the
+// CDNA4 matrix cores (v_mfma_f32_16x16x32_bf16, FP32 accumulation), one
+// 16-row panel of C_i per wave.
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+#include "common.hpp"
+
+namespace vbh {
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_norms(const T* __restrict__ vals, const int64_t* __restrict__ voff,
+                                               int64_t nnzb, double* __restrict__ out) {
+    const int64_t k = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (k >= nnzb) return;
+    double s = 0.0;
+    for (int64_t i = voff[k] + lane; i < voff[k + 1]; i += 64) s += sqabs(vals[i]);
+    for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+    if (lane == 0) out[k] = sqrt(s);
+}
+
+struct SymArgs {
+    const int* a_row_ptr;
+    const int* a_col_ind;
+    const double* a_norm;
+    const int* b_row_ptr;
+    const int* b_col_ind;
+    const double* b_norm;
+    const int* rows;        // rows handled by this launch
+    int nrows;
+    const int* tsize;       // hash table size per row (power of two)
+    const int64_t* out_off; // per row: start of its kept-column region
+    int* out_cols;          // kept columns (unsorted)
+    int* out_count;         // per row count
+    int64_t* pairs;         // per row multiplication paths
+    double thr;
+    int* g_keys;            // global tables (LDS == false)
+    double* g_vals;
+    const int64_t* g_off;   // per launch-row table offset
+};
+
+template <bool LDS>
+__global__ __launch_bounds__(256) void k_symbolic(SymArgs a) {
+    extern __shared__ char smem[];
+    const int q = blockIdx.x;
+    if (q >= a.nrows) return;
+    const int i = a.rows[q];
+    const int S = a.tsize[i];
+    int* keys;
+    double* vals;
+    if constexpr (LDS) {
+        vals = reinterpret_cast<double*>(smem);
+        keys = reinterpret_cast<int*>(smem + sizeof(double) * S);
+    } else {
+        keys = a.g_keys + a.g_off[q];
+        vals = a.g_vals + a.g_off[q];
+    }
+    for (int t = threadIdx.x; t < S; t += blockDim.x) {
+        keys[t] = -1;
+        vals[t] = 0.0;
+    }
+    __syncthreads();
+    const int mask = S - 1;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int64_t npairs = 0;
+    for (int ka = a.a_row_ptr[i] + wave; ka < a.a_row_ptr[i + 1]; ka += 4) {
+        const int k = a.a_col_ind[ka];
+        const double na = a.a_norm[ka];
+        const int b0 = a.b_row_ptr[k], b1 = a.b_row_ptr[k + 1];
+        npairs += b1 - b0;
+        for (int kb = b0 + lane; kb < b1; kb += 64) {
+            const int j = a.b_col_ind[kb];
+            const double v = na * a.b_norm[kb];
+            int h = j & mask;
+            while (true) {
+                const int prev = atomicCAS(&keys[h], -1, j);
+                if (prev == -1 || prev == j) {
+                    atomicAdd(&vals[h], v);
+                    break;
+                }
+                h = (h + 1) & mask;
+            }
+        }
+    }
+    __syncthreads();
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    int* out = a.out_cols + a.out_off[i];
+    for (int t = threadIdx.x; t < S; t += blockDim.x) {
+        if (keys[t] >= 0 && vals[t] > a.thr) {
+            const int pos = atomicAdd(&cnt, 1);
+            out[pos] = keys[t];
+        }
+    }
+    if (lane == 0) atomicAdd((unsigned long long*)&a.pairs[i], (unsigned long long)npairs);
+    __syncthreads();
+    if (threadIdx.x == 0) a.out_count[i] = cnt;
+}
+
+template <typename T, typename CT>
+struct NumArgs {
+    const int* a_row_ptr;
+    const int* a_col_ind;
+    const int64_t* a_voff;
+    const T* a_vals;
+    const double* a_norm;
+    const int* a_rbs;
+    const int* a_cbs;  // = B's row block sizes
+    const int* b_row_ptr;
+    const int* b_col_ind;
+    const int64_t* b_voff;
+    const T* b_vals;
+    const double* b_norm;
+    const int* c_row_ptr;
+    const int* c_col_ind;
+    const int64_t* c_voff;
+    const int* c_cbs;
+    T* c_vals;
+    unsigned long long* npairs;
+    double thr;
+};
+
+__device__ inline int find_col(const int* __restrict__ cols, int lo, int hi, int j) {
+    // first position in [lo, hi) with cols[pos] >= j; -1 if absent
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (cols[mid] < j) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <typename T>
+__device__ inline T to_ct(double x) { return T(x); }
+
+// CT: the product/accumulation type (T itself, or float / c64).
+template <typename T, typename CT>
+__device__ inline CT narrow(T x) {
+    if constexpr (std::is_same_v<T, CT>) return x;
+    else if constexpr (Sc<T>::is_complex) return CT{float(x.re), float(x.im)};
+    else return CT(x);
+}
+template <typename T, typename CT>
+__device__ inline T widen(CT x) {
+    if constexpr (std::is_same_v<T, CT>) return x;
+    else if constexpr (Sc<T>::is_complex) return T{double(x.re), double(x.im)};
+    else return T(x);
+}
+
+template <typename T, typename CT>
+__global__ __launch_bounds__(256) void k_numeric(NumArgs<T, CT> a) {
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int a0 = a.a_row_ptr[i], a1 = a.a_row_ptr[i + 1];
+    const int c0 = a.c_row_ptr[i], c1 = a.c_row_ptr[i + 1];
+    if (c0 == c1) return;
+    const int r = a.a_rbs[i];
+    const int row_count = a1 - a0;
+    const double row_eps = a.thr / double(row_count > 1 ? row_count : 1);
+    unsigned long long used = 0;
+    for (int ka = a0; ka < a1; ++ka) {
+        const int k = a.a_col_ind[ka];
+        const int m = a.a_cbs[k];
+        const double na = a.a_norm[ka];
+        const T* A = a.a_vals + a.a_voff[ka];
+        for (int kb = a.b_row_ptr[k] + wave; kb < a.b_row_ptr[k + 1]; kb += 4) {
+            if (na * a.b_norm[kb] < row_eps) continue;
+            const int j = a.b_col_ind[kb];
+            const int pos = find_col(a.c_col_ind, c0, c1, j);
+            if (pos >= c1 || a.c_col_ind[pos] != j) continue;
+            ++used;
+            const int c = a.c_cbs[j];
+            const T* B = a.b_vals + a.b_voff[kb];
+            T* C = a.c_vals + a.c_voff[pos];
+            for (int e = lane; e < r * c; e += 64) {
+                const int ra = e / c, cb = e - ra * c;
+                CT acc = narrow<T, CT>(C[e]);
+                for (int q = 0; q < m; ++q)
+                    acc = Sc<CT>::fma(narrow<T, CT>(A[ra * m + q]), narrow<T, CT>(B[int64_t(q) * c + cb]), acc);
+                C[e] = widen<T, CT>(acc);
+            }
+        }
+        __syncthreads();
+    }
+    if (lane == 0 && used) atomicAdd(&a.npairs[0], used);
+}
+
+// ------------------------------------------------------------ BF16 MFMA numeric
+// Real matrices only.  The output panel of one C row i is r x (sum_j c_j);
+// each A slot k contributes A_ik (r x m) times the row panel of B_k restricted
+// to C's columns.  A wave owns 16 output columns of the panel: for every A
+// slot it gathers the matching B_kj columns (zero where C_ij is absent or the
+// pair is skipped), pads the inner dimension m to 32 and the r rows to 16,
+// and issues one v_mfma_f32_16x16x32_bf16.
+typedef __attribute__((__vector_size__(8 * sizeof(short)))) short bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+__device__ inline short to_bf16(double x) {
+    // round-to-nearest-even on the FP32 value
+    float f = float(x);
+    unsigned u = __float_as_uint(f);
+    const unsigned lsb = (u >> 16) & 1u;
+    u += 0x7fffu + lsb;
+    return short(u >> 16);
+}
+
+struct MfmaArgs {
+    const int* a_row_ptr;
+    const int* a_col_ind;
+    const int64_t* a_voff;
+    const double* a_vals;
+    const double* a_norm;
+    const int* a_rbs;
+    const int* a_cbs;
+    const int* b_row_ptr;
+    const int* b_col_ind;
+    const int64_t* b_voff;
+    const double* b_vals;
+    const double* b_norm;
+    const int* c_row_ptr;
+    const int* c_col_ind;
+    const int64_t* c_voff;
+    const int* c_cbs;
+    const int* c_colstart;  // per C slot: first panel column of the block within its row
+    const int* panel_width; // per C row: total panel columns
+    double* c_vals;
+    double thr;
+};
+
+// Panel column -> (C slot, column within block), found by binary search over
+// c_colstart of the row.
+__device__ inline int slot_of_panel_col(const int* __restrict__ colstart, int c0, int c1, int pc) {
+    int lo = c0, hi = c1 - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (colstart[mid] <= pc) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ __launch_bounds__(256) void k_numeric_mfma_bf16(MfmaArgs a) {
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int a0 = a.a_row_ptr[i], a1 = a.a_row_ptr[i + 1];
+    const int c0 = a.c_row_ptr[i], c1 = a.c_row_ptr[i + 1];
+    if (c0 == c1) return;
+    const int r = a.a_rbs[i];
+    const int width = a.panel_width[i];
+    const int row_count = a1 - a0;
+    const double row_eps = a.thr / double(row_count > 1 ? row_count : 1);
+    // MFMA 16x16x32 bf16 operand layout (per lane): A operand lane l holds
+    // row (l % 16), k-elements 8*(l / 16) .. +7; B operand lane l holds column
+    // (l % 16), k-elements 8*(l / 16) .. +7; result lane l holds column
+    // (l % 16), rows 4*(l / 16) .. +3.
+    const int lr = lane & 15;
+    const int lk = (lane >> 4) * 8;
+    for (int pc0 = wave * 16; pc0 < width; pc0 += 64) {
+        const int pc = pc0 + lr;  // this lane's panel column (B operand / result)
+        int slot = -1, jcol = -1, cb = 0, cwidth = 0;
+        if (pc < width) {
+            slot = slot_of_panel_col(a.c_colstart, c0, c1, pc);
+            jcol = a.c_col_ind[slot];
+            cb = pc - a.c_colstart[slot];
+            cwidth = a.c_cbs[jcol];
+        }
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int ka = a0; ka < a1; ++ka) {
+            const int k = a.a_col_ind[ka];
+            const int m = a.a_cbs[k];
+            const double na = a.a_norm[ka];
+            const double* A = a.a_vals + a.a_voff[ka];
+            // this lane's B_kj column, if block (k, jcol) exists and survives the skip
+            const double* Bcol = nullptr;
+            if (slot >= 0) {
+                const int b0 = a.b_row_ptr[k], b1 = a.b_row_ptr[k + 1];
+                const int pb = find_col(a.b_col_ind, b0, b1, jcol);
+                if (pb < b1 && a.b_col_ind[pb] == jcol && !(na * a.b_norm[pb] < row_eps))
+                    Bcol = a.b_vals + a.b_voff[pb] + cb;
+            }
+            for (int kk = 0; kk < m; kk += 32) {
+                bf16x8 av, bv;
+#pragma unroll
+                for (int t = 0; t < 8; ++t) {
+                    const int q = kk + lk + t;
+                    av[t] = (lr < r && q < m) ? to_bf16(A[lr * m + q]) : short(0);
+                    bv[t] = (Bcol && q < m) ? to_bf16(Bcol[int64_t(q) * cwidth]) : short(0);
+                }
+                acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, acc, 0, 0, 0);
+            }
+        }
+        if (slot >= 0) {
+            double* C = a.c_vals + a.c_voff[slot];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int row = (lane >> 4) * 4 + t;
+                if (row < r) C[row * cwidth + cb] = double(acc[t]);
+            }
+        }
+    }
+}
+
+static inline unsigned nblk4(int64_t n) { return unsigned((n + 3) / 4); }
+
+}  // namespace vbh
+
+using namespace vbh;
+
+extern "C" VBH_API int vbh_spgemm(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric, vbh_matrix_t* out,
+                                  int64_t* stats, void* stream) {
+    return guard([&] {
+        VBH_REQUIRE(A && B && out, "null argument");
+        VBH_REQUIRE(A->dtype == B->dtype, "A and B must have the same dtype");
+        VBH_REQUIRE(A->nbc == B->nbr, "spmm: A's local columns must be B's block rows");
+        for (int k = 0; k < B->nbr; ++k)
+            VBH_REQUIRE(A->h_cbs[k] == B->h_rbs[k], "spmm: inner block sizes differ");
+        VBH_REQUIRE(numeric >= 0 && numeric <= 2, "numeric must be 0, 1 or 2");
+        VBH_REQUIRE(!(numeric == 2 && A->dtype != VBH_F64), "BF16 MFMA numeric supports real matrices");
+        VBH_REQUIRE(!(numeric == 2 && A->max_block > 16), "BF16 MFMA numeric supports blocks up to 16");
+        hipStream_t s = (hipStream_t)stream;
+        const int nbr = A->nbr;
+        // ---- norms
+        double* an = (double*)dalloc(sizeof(double) * std::max<int64_t>(A->nnzb, 1));
+        double* bn = (double*)dalloc(sizeof(double) * std::max<int64_t>(B->nnzb, 1));
+        auto run_norms = [&](vbh_matrix* M, double* o) {
+            if (M->nnzb == 0) return;
+            if (M->dtype == VBH_F64)
+                hipLaunchKernelGGL(k_norms<double>, dim3(nblk4(M->nnzb)), dim3(256), 0, s, (const double*)M->vals,
+                                   M->voff, M->nnzb, o);
+            else
+                hipLaunchKernelGGL(k_norms<c128>, dim3(nblk4(M->nnzb)), dim3(256), 0, s, (const c128*)M->vals,
+                                   M->voff, M->nnzb, o);
+            VBH_CHECK_HIP(hipGetLastError());
+        };
+        run_norms(A, an);
+        run_norms(B, bn);
+        // ---- symbolic sizing (host)
+        std::vector<int64_t> bound(nbr, 0), out_off(nbr + 1, 0);
+        std::vector<int> tsize(nbr, 1);
+        for (int i = 0; i < nbr; ++i) {
+            int64_t b = 0;
+            for (int ka = A->h_row_ptr[i]; ka < A->h_row_ptr[i + 1]; ++ka) {
+                const int k = A->h_col_ind[ka];
+                b += B->h_row_ptr[k + 1] - B->h_row_ptr[k];
+            }
+            bound[i] = b;
+            const int64_t cap = std::min<int64_t>(b, B->nbc);
+            out_off[i + 1] = out_off[i] + cap;
+            int64_t S = 1;
+            while (S < 2 * cap + 1) S <<= 1;
+            VBH_REQUIRE(S <= (int64_t(1) << 30), "symbolic hash table too large");
+            tsize[i] = int(S);
+        }
+        int* d_tsize = upload(tsize, s);
+        int64_t* d_out_off = upload(out_off, s);
+        int* d_out_cols = (int*)dalloc(sizeof(int) * std::max<int64_t>(out_off[nbr], 1));
+        int* d_count = (int*)dalloc(sizeof(int) * std::max(nbr, 1));
+        int64_t* d_pairs = (int64_t*)dalloc(sizeof(int64_t) * std::max(nbr, 1));
+        VBH_CHECK_HIP(hipMemsetAsync(d_pairs, 0, sizeof(int64_t) * std::max(nbr, 1), s));
+        VBH_CHECK_HIP(hipMemsetAsync(d_count, 0, sizeof(int) * std::max(nbr, 1), s));
+        SymArgs sa{};
+        sa.a_row_ptr = A->row_ptr;
+        sa.a_col_ind = A->col_ind;
+        sa.a_norm = an;
+        sa.b_row_ptr = B->row_ptr;
+        sa.b_col_ind = B->col_ind;
+        sa.b_norm = bn;
+        sa.tsize = d_tsize;
+        sa.out_off = d_out_off;
+        sa.out_cols = d_out_cols;
+        sa.out_count = d_count;
+        sa.pairs = d_pairs;
+        sa.thr = thr;
+        // LDS classes: <=256, <=1024, <=4096 slots; wider rows use global tables
+        const int classes[3] = {256, 1024, 4096};
+        std::vector<int> rows_cls[4];
+        for (int i = 0; i < nbr; ++i) {
+            if (bound[i] == 0) continue;
+            int c = 3;
+            for (int q = 0; q < 3; ++q)
+                if (tsize[i] <= classes[q]) { c = q; break; }
+            rows_cls[c].push_back(i);
+        }
+        std::vector<void*> tmp_free;
+        for (int q = 0; q < 3; ++q) {
+            if (rows_cls[q].empty()) continue;
+            int* d_rows = upload(rows_cls[q], s);
+            tmp_free.push_back(d_rows);
+            sa.rows = d_rows;
+            sa.nrows = int(rows_cls[q].size());
+            const size_t shm = size_t(classes[q]) * (sizeof(double) + sizeof(int));
+            hipLaunchKernelGGL(k_symbolic<true>, dim3(sa.nrows), dim3(256), shm, s, sa);
+            VBH_CHECK_HIP(hipGetLastError());
+        }
+        if (!rows_cls[3].empty()) {
+            // global tables, processed in batches bounded to ~1 GiB of tables
+            const auto& rows = rows_cls[3];
+            size_t pos = 0;
+            while (pos < rows.size()) {
+                std::vector<int64_t> goff(1, 0);
+                std::vector<int> batch;
+                while (pos < rows.size() && (batch.empty() || goff.back() + tsize[rows[pos]] <= (int64_t(1) << 26))) {
+                    batch.push_back(rows[pos]);
+                    goff.push_back(goff.back() + tsize[rows[pos]]);
+                    ++pos;
+                }
+                int* d_rows = upload(batch, s);
+                int64_t* d_goff = upload(goff, s);
+                int* gk = (int*)dalloc(sizeof(int) * goff.back());
+                double* gv = (double*)dalloc(sizeof(double) * goff.back());
+                sa.rows = d_rows;
+                sa.nrows = int(batch.size());
+                sa.g_keys = gk;
+                sa.g_vals = gv;
+                sa.g_off = d_goff;
+                hipLaunchKernelGGL(k_symbolic<false>, dim3(sa.nrows), dim3(256), 0, s, sa);
+                VBH_CHECK_HIP(hipGetLastError());
+                VBH_CHECK_HIP(hipStreamSynchronize(s));
+                for (void* p : {(void*)d_rows, (void*)d_goff, (void*)gk, (void*)gv}) dfree(p);
+            }
+        }
+        // ---- pattern of C (host sort per row)
+        std::vector<int> counts(nbr);
+        std::vector<int> cols(out_off[nbr]);
+        std::vector<int64_t> pairs(nbr);
+        VBH_CHECK_HIP(hipMemcpyAsync(counts.data(), d_count, sizeof(int) * nbr, hipMemcpyDeviceToHost, s));
+        if (!cols.empty())
+            VBH_CHECK_HIP(hipMemcpyAsync(cols.data(), d_out_cols, sizeof(int) * cols.size(), hipMemcpyDeviceToHost, s));
+        VBH_CHECK_HIP(hipMemcpyAsync(pairs.data(), d_pairs, sizeof(int64_t) * nbr, hipMemcpyDeviceToHost, s));
+        VBH_CHECK_HIP(hipStreamSynchronize(s));
+        for (void* p : tmp_free) dfree(p);
+        for (void* p : {(void*)d_tsize, (void*)d_out_off, (void*)d_out_cols, (void*)d_count, (void*)d_pairs}) dfree(p);
+        std::vector<int> crp(nbr + 1, 0), cci;
+        int64_t total = 0;
+        for (int i = 0; i < nbr; ++i) total += counts[i];
+        VBH_REQUIRE(total <= int64_t(INT32_MAX), "SpGEMM result exceeds 2^31 blocks");
+        cci.reserve(total);
+        for (int i = 0; i < nbr; ++i) {
+            const int64_t b = out_off[i];
+            std::sort(cols.begin() + b, cols.begin() + b + counts[i]);
+            cci.insert(cci.end(), cols.begin() + b, cols.begin() + b + counts[i]);
+            crp[i + 1] = int(cci.size());
+        }
+        vbh_matrix* C = build_matrix(nbr, B->nbc, A->h_rbs.data(), B->h_cbs.data(), crp.data(), cci.data(), A->dtype, s);
+        unsigned long long* d_np = (unsigned long long*)dalloc(sizeof(unsigned long long));
+        VBH_CHECK_HIP(hipMemsetAsync(d_np, 0, sizeof(unsigned long long), s));
+        // ---- numeric
+        if (C->nnzb > 0) {
+            if (numeric == 2) {
+                std::vector<int> colstart(C->nnzb), width(nbr);
+                for (int i = 0; i < nbr; ++i) {
+                    int w = 0;
+                    for (int q = crp[i]; q < crp[i + 1]; ++q) {
+                        colstart[q] = w;
+                        w += B->h_cbs[cci[q]];
+                    }
+                    width[i] = w;
+                }
+                int* d_cs = upload(colstart, s);
+                int* d_w = upload(width, s);
+                MfmaArgs ma{A->row_ptr, A->col_ind, A->voff, (const double*)A->vals, an, A->rbs, A->cbs,
+                            B->row_ptr, B->col_ind, B->voff, (const double*)B->vals, bn, C->row_ptr, C->col_ind,
+                            C->voff, C->cbs, d_cs, d_w, (double*)C->vals, thr};
+                hipLaunchKernelGGL(k_numeric_mfma_bf16, dim3(nbr), dim3(256), 0, s, ma);
+                VBH_CHECK_HIP(hipGetLastError());
+                VBH_CHECK_HIP(hipStreamSynchronize(s));
+                dfree(d_cs);
+                dfree(d_w);
+            } else {
+                auto launch = [&](auto* tag, auto* ctag) {
+                    using T = std::remove_pointer_t<decltype(tag)>;
+                    using CT = std::remove_pointer_t<decltype(ctag)>;
+                    NumArgs<T, CT> na{A->row_ptr, A->col_ind, A->voff, (const T*)A->vals, an, A->rbs, A->cbs,
+                                      B->row_ptr, B->col_ind, B->voff, (const T*)B->vals, bn, C->row_ptr,
+                                      C->col_ind, C->voff, C->cbs, (T*)C->vals, d_np, thr};
+                    hipLaunchKernelGGL((k_numeric<T, CT>), dim3(nbr), dim3(256), 0, s, na);
+                    VBH_CHECK_HIP(hipGetLastError());
+                };
+                if (A->dtype == VBH_F64) {
+                    if (numeric == 0) launch((double*)nullptr, (double*)nullptr);
+                    else launch((double*)nullptr, (float*)nullptr);
+                } else {
+                    if (numeric == 0) launch((c128*)nullptr, (c128*)nullptr);
+                    else launch((c128*)nullptr, (c64*)nullptr);
+                }
+            }
+        }
+        unsigned long long np = 0;
+        VBH_CHECK_HIP(hipMemcpyAsync(&np, d_np, sizeof(np), hipMemcpyDeviceToHost, s));
+        VBH_CHECK_HIP(hipStreamSynchronize(s));
+        dfree(d_np);
+        dfree(an);
+        dfree(bn);
+        const int64_t before = C->nnzb;
+        // ---- filter
+        vbh_matrix* F = nullptr;
+        if (thr > 0.0 && C->nnzb > 0) {
+            const int rc = vbh_matrix_filter(C, thr, &F, stream);
+            if (rc != VBH_OK) {
+                destroy_matrix(C);
+                throw Error(rc, vbh_last_error());
+            }
+        }
+        if (F) {
+            destroy_matrix(C);
+            C = F;
+        }
+        if (stats) {
+            int64_t sp = 0;
+            for (int64_t p : pairs) sp += p;
+            stats[0] = sp;
+            stats[1] = int64_t(np);
+            stats[2] = before;
+            stats[3] = C->nnzb;
+        }
+        *out = C;
+    });
+}

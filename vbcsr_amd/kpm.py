@@ -1,0 +1,79 @@
+"""Native KPM: moments, batched moments and the DOS reconstruction.
+
+The reference computes KPM moments in user-level Python loops over
+``H @ V``, ``bdot``, ``axpby`` and ``scale``
+(examples/ex08_graphene_dos.py:66-156, examples/inp_nanocrystal_dos/kpm.py:
+46-164).  Here the whole recursion runs on the device: one fused kernel per
+step (SpMM + three-term update + FP64 dot partials), optionally two moments
+per matvec, optionally in FP32 with FP64 reductions, and -- for many
+Hamiltonians -- one launch per step over the whole batch, captured in one HIP
+graph.
+"""
+from __future__ import annotations
+
+from typing import Sequence, Tuple
+
+import numpy as np
+
+from . import _core
+from .matrix import VBCSR
+from .multivector import DistMultiVector
+from .utils import jackson_kernel
+
+_MODES = {"plain": 0, "one": 1, "two": 2}
+_PREC = {"fp64": 0, "fp32": 1}
+
+
+def kpm_moments(H: VBCSR, V0: DistMultiVector, num_moments: int, mode: str = "two",
+                precision: str = "fp64", center: float = 0.0, half_width: float = 1.0) -> np.ndarray:
+    """mu_n = sum_v Re <v0_v, T_n(H~) v0_v>, n < num_moments, H~ = (H - center)/half_width.
+
+    mode "plain": separate SpMM / axpby / dot kernels, one moment per matvec
+    (the unfused reference composition, kept as a baseline); "one": fused,
+    one moment per matvec; "two": fused, two moments per matvec (needs a
+    Hermitian H).  precision "fp32" runs the recursion in FP32 with FP64 dot
+    products.  Columns beyond 64 are processed in groups of 64."""
+    return np.asarray(H._core.kpm_moments(V0._core, int(num_moments), _MODES[mode], float(center),
+                                          float(half_width), _PREC[precision]))
+
+
+def kpm_moments_batched(Hs: Sequence[VBCSR], V0s: Sequence[DistMultiVector], num_moments: int,
+                        mode: str = "two", precision: str = "fp64", center: float = 0.0,
+                        half_width: float = 1.0, graph: bool = True) -> np.ndarray:
+    """Moments of many independent Hamiltonians (rows: matrices), one kernel
+    launch per recursion step over the whole batch, steps in one HIP graph."""
+    mats = [h._core for h in Hs]
+    vecs = [v._core for v in V0s]
+    return np.asarray(_core.kpm_moments_batched(mats, vecs, int(num_moments), _MODES[mode], float(center),
+                                                float(half_width), _PREC[precision], bool(graph)))
+
+
+def chebyshev_nodes(num_points: int) -> np.ndarray:
+    """Ascending Gauss-Chebyshev nodes x_k = -cos(pi (k + 1/2)/K); never +-1."""
+    return -np.cos(np.pi * (np.arange(num_points) + 0.5) / num_points)
+
+
+def dos_from_moments(moments: np.ndarray, num_points: int = 2000, kernel: str = "jackson",
+                     x: np.ndarray = None) -> Tuple[np.ndarray, np.ndarray]:
+    """rho(x) = [g_0 mu_0 + 2 sum_n g_n mu_n T_n(x)] / (pi sqrt(1 - x^2)) on
+    Gauss-Chebyshev nodes (examples/ex08_graphene_dos.py:145-155 with the node
+    choice of examples/inp_nanocrystal_dos/kpm.py:135-151)."""
+    mu = np.asarray(moments, dtype=np.float64)
+    M = mu.shape[0]
+    g = jackson_kernel(M) if kernel == "jackson" else np.ones(M)
+    if x is None:
+        x = chebyshev_nodes(num_points)
+    t0 = np.ones_like(x)
+    rho = g[0] * mu[0] * t0
+    if M > 1:
+        t1 = x.copy()
+        rho = rho + 2.0 * g[1] * mu[1] * t1
+        for n in range(2, M):
+            t0, t1 = t1, 2.0 * x * t1 - t0
+            rho = rho + 2.0 * g[n] * mu[n] * t1
+    return x, rho / (np.pi * np.sqrt(1.0 - x * x))
+
+
+def dos_error(rho: np.ndarray, rho_ref: np.ndarray) -> float:
+    """max_x |rho - rho_ref| / max_x rho_ref (both on the same interior nodes)."""
+    return float(np.max(np.abs(rho - rho_ref)) / np.max(np.abs(rho_ref)))
