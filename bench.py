@@ -1,0 +1,296 @@
+"""Benchmark of the variable-block sparse hot path on one MI355X.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--quick]
+
+Workload (the configuration the metric is quoted on): graphene-like
+tight-binding Hamiltonian, periodic honeycomb lattice of 200 x 200 cells, 4
+orbitals per atom (4 x 4 blocks, N = 320,000 rows, 800,000 blocks), on-site +
+nearest + next-nearest neighbour blocks with N(0,1) entries / distance,
+spectrum scaled to [-0.99, 0.99]; KPM density of states with 200 moments,
+Jackson kernel, 10 Rademacher vectors as one block of columns.
+
+A step is one full KPM moment computation (200 moments x 10 vectors) with the
+matrix and vectors resident in HBM.  ``value`` is the FP32-mixed throughput
+(FP32 recursion and matrix values, FP64 moment reductions, fused kernel, two
+moments per matvec) in moment-vecs/s; the same line carries the plain FP64
+GPU path (separate SpMM / axpby / dot kernels, one moment per matvec), the
+fused FP64 path, the DOS error of FP32 against FP64, block-SpMV bandwidth on a
+1.5 GB variable-block matrix (beyond the 256 MB Infinity Cache), the batched
+many-Hamiltonian KPM, the roofline of the dominant kernel and the reference
+CPU path timed on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
+    p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--quick", action="store_true", help="smaller lattice (smoke of the bench itself)")
+    p.add_argument("--only", default="", help="comma list: kpm,spmv,batched,cpu (default all)")
+    return p.parse_args()
+
+
+def timed(fn, steps, warmup, sync):
+    for _ in range(warmup):
+        fn()
+    sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    sync()
+    return (time.perf_counter() - t0) / steps
+
+
+def spmm_bytes(A, K, p_val, p_vec, with_z=False, with_w=False):
+    """Algorithmic bytes of one SpMM / fused step: values, block indices,
+    row pointers, X gathered once, Y written once (+ Z, W reads)."""
+    n = A.n
+    b = p_val * A.nnz + 4 * A.nnzb + 4 * (A.nb + 1) + p_vec * K * (n + n)
+    if with_z:
+        b += p_vec * K * n
+    if with_w:
+        b += p_vec * K * n
+    return b
+
+
+def read_traffic(name):
+    """HBM bytes per launch measured by rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, calibrated on a copy kernel), committed under profiles/."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(G, K, M, seed):
+    """Reference vbcsr C++ core (oracle/_ref) on this host: the ex08 KPM
+    composition (mult_dense + scale + axpy + bdot), FP64, OpenMP threads."""
+    import oracle
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    if oracle.ref_available():
+        m = oracle.Mat.of(G)
+        R = oracle.RefMat(m)
+        V0 = R.random_mv(K, seed, False)
+        R.kpm_moments(V0, 3)  # warm-up: builds the cached apply plan
+        mu, secs = R.kpm_moments(V0, M)
+        return {"value": M * K / secs, "unit": "moment-vecs/s", "cores": threads, "kind": "reference",
+                "sample": f"reference vbcsr C++ core (oracle/_ref, -O3, OpenMP, serial MPI_COMM_NULL) on the "
+                          f"same graphene 200x200 Hamiltonian: ex08 KPM composition, {M} moments x {K} vectors, "
+                          f"FP64, {secs:.2f} s"}, mu
+    m = oracle.Mat.of(G)
+    V0 = oracle.rademacher(G.bs, np.arange(G.nb), K, seed, False, G.n)
+    Ms = min(M, 12)
+    t0 = time.perf_counter()
+    mu = oracle.kpm_moments(m, V0, Ms, mode=1)
+    secs = time.perf_counter() - t0
+    return {"value": Ms * K / secs, "unit": "moment-vecs/s", "cores": 1, "kind": "port",
+            "sample": f"CPU restatement (oracle/, 1 thread), {Ms} moments x {K} vectors, FP64, {secs:.2f} s"}, None
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    only = set(filter(None, args.only.split(",")))
+    want = lambda k: not only or k in only  # noqa: E731
+
+    import torch
+
+    import vbcsr_amd
+    from vbcsr_amd import VBCSR
+    from vbcsr_amd import kpm as KP
+    from vbcsr_amd import workloads as W
+
+    sync = torch.cuda.synchronize
+    nx = 60 if args.quick else 200
+    M, K, seed = 200, 10, 42
+    out = {}
+
+    log(f"generating graphene {nx}x{nx} x4 orbitals")
+    G = W.graphene(nx, nx, 4, seed=0)
+    H = VBCSR.from_block_csr(G.bs, G.row_ptr, G.col_ind, G.vals)
+    V0 = H.create_multivector(K)
+    V0.set_random(seed, normalize=False)
+    sync()
+
+    ms_step = None
+    roofline = None
+    if want("kpm"):
+        runs = {}
+        for name, mode, prec in (("fp64_plain", "plain", "fp64"), ("fp64_fused", "two", "fp64"),
+                                 ("fp32_mixed", "two", "fp32")):
+            res = {}
+
+            def step(mode=mode, prec=prec, res=res):
+                res["mu"] = KP.kpm_moments(H, V0, M, mode=mode, precision=prec)
+
+            secs = timed(step, args.steps, args.warmup, sync)
+            kms, launches = vbcsr_amd._core.kpm_last_step_time()
+            runs[name] = dict(secs=secs, mu=res["mu"], kernel_ms=kms, launches=launches)
+            log(f"{name}: {secs * 1e3:.3f} ms/step, {M * K / secs:.4g} moment-vecs/s, "
+                f"step kernels {kms:.3f} ms / {launches} launches")
+        _, rho64 = KP.dos_from_moments(runs["fp64_plain"]["mu"])
+        _, rho32 = KP.dos_from_moments(runs["fp32_mixed"]["mu"])
+        _, rho64f = KP.dos_from_moments(runs["fp64_fused"]["mu"])
+        dos_err = KP.dos_error(rho32, rho64)
+        fp32 = runs["fp32_mixed"]
+        ms_step = fp32["secs"] * 1e3
+        out["value"] = M * K / fp32["secs"]
+        out["fp64_plain_moment_vecs_per_s"] = M * K / runs["fp64_plain"]["secs"]
+        out["fp64_fused_moment_vecs_per_s"] = M * K / runs["fp64_fused"]["secs"]
+        out["speedup_fp32_mixed_vs_fp64_plain"] = runs["fp64_plain"]["secs"] / fp32["secs"]
+        out["dos_error_fp32_vs_fp64"] = dos_err
+        out["dos_error_fp64_fused_vs_fp64_plain"] = KP.dos_error(rho64f, rho64)
+        out["moments_per_matvec"] = {"fp64_plain": 1, "fp64_fused": 2, "fp32_mixed": 2}
+        # roofline of the dominant kernel: the fused FP32 step (k_apply_rows<float,4,true>)
+        per_launch_ms = fp32["kernel_ms"] / max(fp32["launches"], 1)
+        alg = spmm_bytes(G, K, 4, 4, with_z=True)
+        achieved = alg / (per_launch_ms * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK_GBS,
+                    "traffic": read_traffic("pmc_kpm_step_fp32_graphene200.txt"),
+                    "kernel": "k_apply_rows<float,4,fused> (one KPM step)",
+                    "alg_bytes_per_launch": alg, "avg_launch_us": per_launch_ms * 1e3,
+                    "note": "working set ~95 MB fits the 256 MB Infinity Cache; see spmv_large for HBM"}
+        p64 = runs["fp64_fused"]
+        pl64 = p64["kernel_ms"] / max(p64["launches"], 1)
+        a64 = spmm_bytes(G, K, 8, 8, with_z=True)
+        out["fp64_fused_step_gbs"] = a64 / (pl64 * 1e-3) / 1e9
+
+    if want("spmv"):
+        # block SpMV FP64 on a 1.5 GB variable-block matrix (>> 256 MB MALL)
+        nb = 20000 if args.quick else 130000
+        log(f"generating random VBCSR nb={nb} for SpMV bandwidth")
+        B = W.random_vbcsr(nb, per_row=40, seed=1)
+        HB = VBCSR.from_block_csr(B.bs, B.row_ptr, B.col_ind, B.vals)
+        x = torch.rand(B.n, dtype=torch.float64, device="cuda")
+        y = torch.zeros(B.n, dtype=torch.float64, device="cuda")
+        core = HB._core
+        core.spmm_raw(1, x.data_ptr(), 1, y.data_ptr(), 1, 0, False)
+        sync()
+        reps = max(args.steps, 20)
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        for _ in range(reps):
+            core.spmm_raw(1, x.data_ptr(), 1, y.data_ptr(), 1, 0, False)
+        ev1.record()
+        sync()
+        ms = ev0.elapsed_time(ev1) / reps
+        bts = 8 * B.nnz + 4 * B.nnzb + 4 * (B.nb + 1) + 8 * B.n * 2
+        gbs = bts / (ms * 1e-3) / 1e9
+        out["spmv_large"] = {"matrix": f"random VBCSR nb={nb}, blocks 4-8, diag + 40 random cols/row, "
+                                       f"{B.nnz * 8 / 1e9:.2f} GB FP64 values",
+                             "ms": ms, "gbs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS, "alg_bytes": bts}
+        log(f"spmv large: {ms:.3f} ms, {gbs:.1f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% of HBM peak)")
+        # the same on the graphene matrix, FP64, K=1 (cache-resident)
+        xg = torch.rand(G.n, dtype=torch.float64, device="cuda")
+        yg = torch.zeros(G.n, dtype=torch.float64, device="cuda")
+        ev0.record()
+        for _ in range(reps):
+            H._core.spmm_raw(1, xg.data_ptr(), 1, yg.data_ptr(), 1, 0, False)
+        ev1.record()
+        sync()
+        msg = ev0.elapsed_time(ev1) / reps
+        bg = 8 * G.nnz + 4 * G.nnzb + 4 * (G.nb + 1) + 8 * G.n * 2
+        out["spmv_graphene"] = {"ms": msg, "gbs": bg / (msg * 1e-3) / 1e9}
+        del HB, x, y
+
+    if want("batched"):
+        count = 16 if args.quick else 64
+        Mb, Kb = 512, 32
+        log(f"generating {count} batched Hamiltonians")
+        Hs = W.batched_hamiltonians(count=count, seed=7, scale=False)
+        Hs = [W.gershgorin_scale(A) for A in Hs]
+        mats = [VBCSR.from_block_csr(A.bs, A.row_ptr, A.col_ind, A.vals) for A in Hs]
+        V0s = []
+        for h, Hm in enumerate(mats):
+            v = Hm.create_multivector(Kb)
+            v.set_random(1000 + h, normalize=False)
+            V0s.append(v)
+        sync()
+        res = {}
+
+        def batched_fp32():
+            res["mu32"] = KP.kpm_moments_batched(mats, V0s, Mb, mode="two", precision="fp32", graph=True)
+
+        def plain_fp64():
+            res["mu64"] = np.stack([KP.kpm_moments(Hm, v, Mb, mode="plain") for Hm, v in zip(mats, V0s)])
+
+        bsteps = max(2, args.steps // 5)
+        t32 = timed(batched_fp32, bsteps, 1, sync)
+        t64 = timed(plain_fp64, max(1, bsteps // 2), 1, sync)
+        errs = []
+        for h in range(count):
+            _, r64 = KP.dos_from_moments(res["mu64"][h])
+            _, r32 = KP.dos_from_moments(res["mu32"][h])
+            errs.append(KP.dos_error(r32, r64))
+        rows = int(sum(A.n for A in Hs))
+        out["batched"] = {"workload": f"{count} random-geometric Hamiltonians, blocks {{1,4,9,13}}, "
+                                      f"{rows} rows total, {Mb} moments x {Kb} vectors each",
+                          "fp32_batched_graph_moment_vecs_per_s": count * Mb * Kb / t32,
+                          "fp64_plain_moment_vecs_per_s": count * Mb * Kb / t64,
+                          "speedup": t64 / t32, "max_dos_error": float(max(errs)),
+                          "nnz_total": int(sum(A.nnz for A in Hs))}
+        log(f"batched: fp32 graph {t32 * 1e3:.2f} ms, fp64 plain {t64 * 1e3:.2f} ms, speedup {t64 / t32:.2f}x, "
+            f"max DOS err {max(errs):.2e}")
+
+    cpu = None
+    if want("cpu") and not args.no_cpu and not args.quick:
+        log("CPU baseline (reference core on the host)")
+        try:
+            cpu, _ = cpu_baseline(G, K, M, seed)
+            log(f"cpu: {cpu['value']:.4g} moment-vecs/s ({cpu['kind']}, {cpu['cores']} threads)")
+        except Exception as e:  # the baseline leg must not hide the GPU result
+            cpu = {"value": None, "unit": "moment-vecs/s", "cores": None, "kind": "reference",
+                   "sample": f"failed: {e}"}
+
+    line = {
+        "metric": "KPM moment-vecs/s + block-SpMV GB/s, FP32-mixed vs FP64 at stated DOS error",
+        "value": out.get("value"),
+        "unit": "moment-vecs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 moment reductions)",
+        "data": "synthetic (seeded graphene-like Hamiltonian, Rademacher vectors)",
+        "config": {"workload": f"graphene {nx}x{nx} cells, 4 orbitals (N={G.n}), KPM {M} moments, "
+                               f"{K} Rademacher vectors, Jackson kernel",
+                   "global_batch": K, "seq_len": M, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    line.update({k: v for k, v in out.items() if k != "value"})
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

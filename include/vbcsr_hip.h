@@ -139,6 +139,10 @@ VBH_API int vbh_kpm_moments_batched(int count, const vbh_matrix_t* ms, int K, co
 /* Chebyshev series Y = sum_n coef_n T_n(H~) X (complex coefficients as
  * interleaved doubles, nterms of them; real matrices use the real parts).
  * examples/ex09_graphene_dynamics.py:102-133, vbcsr/utils.py:71-126. */
+/* HIP-event duration (ms) of the step kernels of the last vbh_kpm_moments or
+ * vbh_kpm_moments_batched call on this process, and the number of kernel
+ * launches (graph nodes) it covers.  Profiling aid; no reference counterpart. */
+VBH_API int vbh_kpm_last_step_time(double* ms, int64_t* launches);
 VBH_API int vbh_cheb_apply(vbh_matrix_t m, int K, const void* X, int ldx, void* Y, int ldy,
                            int nterms, const double* coef, double center, double half_width,
                            int precision, void* stream);

@@ -66,7 +66,10 @@ def test_fp32_mixed_dos_error():
     from vbcsr_amd import kpm as K
     from vbcsr_amd import workloads as W
 
-    A = W.graphene(30, 30, 4, seed=4)
+    # the FP32 rounding errors of the moments average over N*K entries, so the
+    # DOS error falls with the system size; 100x100 cells (N = 80,000) is a
+    # quarter of the benchmark lattice
+    A = W.graphene(100, 100, 4, seed=4)
     H = mk(A)
     V0 = H.create_multivector(10)
     V0.set_random(42, normalize=False)

@@ -15,6 +15,14 @@ _HERE = _os.path.dirname(_os.path.abspath(__file__))
 if not _os.path.exists(_os.path.join(_HERE, "libvbcsr_hip.so")):
     raise ImportError("vbcsr_amd: libvbcsr_hip.so is not built; run `python -m vbcsr_amd.build` first")
 
+try:
+    # Load torch's HIP runtime first when torch is installed: the library then
+    # binds to that same runtime (same soname), so device pointers, streams
+    # and events are shared with torch instead of living in a second runtime.
+    import torch as _torch  # noqa: F401
+except ImportError:  # pragma: no cover - torch is optional for the core path
+    _torch = None
+
 from . import _core  # noqa: E402  (fails loudly if the HIP extension is missing)
 from ._core import AssemblyMode, DistGraph, RedistOp  # noqa: E402
 from .matrix import VBCSR  # noqa: E402

@@ -26,6 +26,8 @@ ARCH = "gfx950"
 
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
+    # code-object v5: loadable by this ROCm's runtime and by the one bundled with torch
+    "-mcode-object-version=5",
     # no packed-FP32 instructions in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wno-unused-result",

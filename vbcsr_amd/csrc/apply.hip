@@ -21,16 +21,43 @@
 
 namespace vbh {
 
-template <typename T, int BS, bool FUSED>
+template <typename T, int BS, int RB, bool FUSED>
 __global__ __launch_bounds__(kThreads) void k_apply_rows(ApplyArgs<T> g) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int chunk = xcd_chunk(g.G);
     double d1 = 0.0, d2 = 0.0;
-    row_loop<T, BS, FUSED>(g, g.split[chunk], g.split[chunk + 1], lane, wave, d1, d2);
+    row_loop<T, BS, RB, FUSED>(g, g.split[chunk], g.split[chunk + 1], lane, wave, d1, d2);
     if constexpr (FUSED) {
         if (g.partials != nullptr) write_partials(g.K, lane, wave, d1, d2, g.partials + int64_t(chunk) * 2 * g.K);
     }
+}
+
+template <typename T, int RPP, bool FUSED>
+__global__ __launch_bounds__(kThreads) void k_apply_spmv(ApplyArgs<T> g) {
+    __shared__ __attribute__((aligned(16))) T tile[kWaves][kSpmvTile(RPP)];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int chunk = xcd_chunk(g.G);
+    double d1 = 0.0, d2 = 0.0;
+    row_loop_spmv<T, RPP, FUSED>(g, g.split[chunk], g.split[chunk + 1], lane, wave, d1, d2, tile[wave]);
+    if constexpr (FUSED) {
+        if (g.partials != nullptr) write_partials(1, lane, wave, d1, d2, g.partials + int64_t(chunk) * 2);
+    }
+}
+
+RowConfig row_config(int K, int max_block, int uniform_bs) {
+    RowConfig c{0, 0, 1};
+    if (K == 1 && max_block <= 8) {
+        c.spmv_rpp = pow2ceil(max_block);
+        return c;
+    }
+    c.bs = uniform_bs == 4 ? 4 : 0;
+    const int Kp = K >= 64 ? 64 : pow2ceil(K);
+    const int NG = 64 / Kp;
+    const int need = (max_block + NG - 1) / NG;
+    c.rb = need <= 1 ? 1 : need <= 2 ? 2 : need <= 4 ? 4 : 8;
+    return c;
 }
 
 // Adjoint: output block column j = sum over incoming (row i, slot k) of
@@ -62,7 +89,7 @@ __global__ __launch_bounds__(kThreads) void k_apply_adjoint(AdjointArgs<T> g) {
             for (int v0 = 0; v0 < K; v0 += Kp) {
                 const int v = v0 + v_l;
                 const bool active = b < c && v < K;
-                T acc = Sc<T>::zero();
+                typename Wide<T>::type acc = Sc<typename Wide<T>::type>::zero();
                 if (active) {
                     for (int q = qb + p_l; q < qe; q += P) {
                         const int i = g.t_row[q];
@@ -71,11 +98,11 @@ __global__ __launch_bounds__(kThreads) void k_apply_adjoint(AdjointArgs<T> g) {
                         const T* blk = g.vals + p.voff[k] + b;
                         const T* xr = g.X + int64_t(p.roff[i]) * g.ldx + v;
                         for (int a = 0; a < r; ++a)
-                            acc = Sc<T>::fma(Sc<T>::conj(blk[int64_t(a) * c]), xr[int64_t(a) * g.ldx], acc);
+                            acc = Sc<typename Wide<T>::type>::fma(widen(Sc<T>::conj(blk[int64_t(a) * c])), widen(xr[int64_t(a) * g.ldx]), acc);
                     }
                 }
                 for (int s = Kp * cpp; s < 64; s <<= 1) acc = acc + shfl_xor(acc, s);
-                if (active && p_l == 0) g.Y[(int64_t(p.coff[j]) + b) * g.ldy + v] = acc;
+                if (active && p_l == 0) g.Y[(int64_t(p.coff[j]) + b) * g.ldy + v] = narrow_to<T>(acc);
             }
         }
     }
@@ -97,15 +124,32 @@ __global__ void k_reduce_partials(const double* __restrict__ partials, int G, in
     if (threadIdx.x == 0) out[e] = accumulate ? out[e] + red[0] : red[0];
 }
 
+template <typename T, int BS, bool FUSED>
+static void launch_spmm_family(int rb, const ApplyArgs<T>& a, hipStream_t s) {
+    const dim3 grid(a.G), block(kThreads);
+    switch (rb) {
+        case 1: hipLaunchKernelGGL((k_apply_rows<T, BS, 1, FUSED>), grid, block, 0, s, a); break;
+        case 2: hipLaunchKernelGGL((k_apply_rows<T, BS, 2, FUSED>), grid, block, 0, s, a); break;
+        case 4: hipLaunchKernelGGL((k_apply_rows<T, BS, 4, FUSED>), grid, block, 0, s, a); break;
+        default: hipLaunchKernelGGL((k_apply_rows<T, BS, 8, FUSED>), grid, block, 0, s, a); break;
+    }
+}
+
 template <typename T, bool FUSED>
 void launch_rows(const vbh_matrix* m, const ApplyArgs<T>& a, hipStream_t s) {
-    const int bs = m->uniform_bs;
+    const RowConfig c = row_config(a.K, m->max_block, m->uniform_bs);
     const dim3 grid(a.G), block(kThreads);
-    switch (bs) {
-        case 2: hipLaunchKernelGGL((k_apply_rows<T, 2, FUSED>), grid, block, 0, s, a); break;
-        case 4: hipLaunchKernelGGL((k_apply_rows<T, 4, FUSED>), grid, block, 0, s, a); break;
-        case 8: hipLaunchKernelGGL((k_apply_rows<T, 8, FUSED>), grid, block, 0, s, a); break;
-        default: hipLaunchKernelGGL((k_apply_rows<T, 0, FUSED>), grid, block, 0, s, a); break;
+    if (c.spmv_rpp > 0) {
+        switch (c.spmv_rpp) {
+            case 1: hipLaunchKernelGGL((k_apply_spmv<T, 1, FUSED>), grid, block, 0, s, a); break;
+            case 2: hipLaunchKernelGGL((k_apply_spmv<T, 2, FUSED>), grid, block, 0, s, a); break;
+            case 4: hipLaunchKernelGGL((k_apply_spmv<T, 4, FUSED>), grid, block, 0, s, a); break;
+            default: hipLaunchKernelGGL((k_apply_spmv<T, 8, FUSED>), grid, block, 0, s, a); break;
+        }
+    } else if (c.bs == 4) {
+        launch_spmm_family<T, 4, FUSED>(c.rb, a, s);
+    } else {
+        launch_spmm_family<T, 0, FUSED>(c.rb, a, s);
     }
     VBH_CHECK_HIP(hipGetLastError());
 }

@@ -176,10 +176,17 @@ struct DevPattern {
     const int* roff;  // nbr+1 scalar row offsets
     const int* coff;  // nbc+1 scalar column offsets
     const int64_t* voff;  // nnzb+1
+    // per slot k of row i: {coff[col_ind[k]], (panel << 8) | c} where c is the
+    // block's column size and panel = sum of c over the row's earlier slots,
+    // so the block's values start at voff[row_ptr[i]] + rbs[i] * panel.  One
+    // lane-parallel load yields every block's x offset, width and values.
+    const int2* meta;
+    const int* row_of;  // n_rows: block row of every scalar row
 };
 
 constexpr int kWaves = 4;  // waves per work-group in the row kernels
 constexpr int kThreads = 64 * kWaves;
+constexpr int kVecPad = 8;  // padding elements after every value array
 
 }  // namespace vbh
 
@@ -195,6 +202,8 @@ struct vbh_matrix {
     int *rbs = nullptr, *cbs = nullptr, *row_ptr = nullptr, *col_ind = nullptr;
     int *roff = nullptr, *coff = nullptr;
     int64_t* voff = nullptr;
+    int2* meta = nullptr;
+    int* row_of = nullptr;
     // values
     void* vals = nullptr;
     void* vals32 = nullptr;  // FP32 copy, built on demand
@@ -208,7 +217,7 @@ struct vbh_matrix {
     bool t_valid = false;
 
     vbh::DevPattern pattern() const {
-        return vbh::DevPattern{nbr, nbc, uniform_bs, rbs, cbs, row_ptr, col_ind, roff, coff, voff};
+        return vbh::DevPattern{nbr, nbc, uniform_bs, rbs, cbs, row_ptr, col_ind, roff, coff, voff, meta, row_of};
     }
     size_t elem_size() const { return dtype == VBH_C128 ? 16 : 8; }
 };

@@ -103,7 +103,7 @@ struct BatchItem {
 // One launch per recursion step over every matrix of the batch: work-group g
 // (XCD-remapped) finds its matrix by binary search over g_base, then runs the
 // shared row loop on its chunk.  step >= 0 produces t_{step+1}.
-template <typename T>
+template <typename T, int RB>
 __global__ __launch_bounds__(kThreads) void k_apply_batched(const BatchItem<T>* __restrict__ items, int nitems,
                                                             int Gtot, int K, int step, double alpha, double beta,
                                                             double gamma0, double gamma, int mode,
@@ -135,9 +135,37 @@ __global__ __launch_bounds__(kThreads) void k_apply_batched(const BatchItem<T>* 
     a.W = mode == 1 ? it.v0 : nullptr;
     a.ldw = K;
     double d1 = 0.0, d2 = 0.0;
-    row_loop<T, 0, true>(a, it.split[chunk], it.split[chunk + 1], lane, wave, d1, d2);
+    row_loop<T, 0, RB, true>(a, it.split[chunk], it.split[chunk + 1], lane, wave, d1, d2);
     write_partials(K, lane, wave, d1, d2, partials + int64_t(gg) * 2 * K);
 }
+
+// HIP-event timing of the last recursion: the step kernels only (setup,
+// the final reduction and the host copy excluded), recorded on the launch
+// stream.  Read through vbh_kpm_last_step_time.
+static double g_last_step_ms = 0.0;
+static int64_t g_last_launches = 0;
+
+struct StepTimer {
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    hipStream_t s;
+    explicit StepTimer(hipStream_t st) : s(st) {
+        VBH_CHECK_HIP(hipEventCreate(&t0));
+        VBH_CHECK_HIP(hipEventCreate(&t1));
+    }
+    void start() { VBH_CHECK_HIP(hipEventRecord(t0, s)); }
+    void stop(int64_t launches) {
+        VBH_CHECK_HIP(hipEventRecord(t1, s));
+        VBH_CHECK_HIP(hipEventSynchronize(t1));
+        float ms = 0.f;
+        VBH_CHECK_HIP(hipEventElapsedTime(&ms, t0, t1));
+        g_last_step_ms = ms;
+        g_last_launches = launches;
+    }
+    ~StepTimer() {
+        if (t0) (void)hipEventDestroy(t0);
+        if (t1) (void)hipEventDestroy(t1);
+    }
+};
 
 template <typename T>
 static int64_t step_count(int M, int mode) {
@@ -206,12 +234,16 @@ static void kpm_single(vbh_matrix* m, int K, const S* V0, int ldv, int M, int mo
     double* partials = (double*)scratch(sizeof(double) * nsteps * G * E, 5);
     const double a1 = 1.0 / half, b1 = -center / half;  // t1 = H~ t0
     const double a2 = 2.0 / half, b2 = -2.0 * center / half;  // t_{n+1} = 2 H~ t_n - t_{n-1}
+    const int64_t rp = std::max<int64_t>(256, (N + G - 1) / G);
+    const int ndot = int((N + rp - 1) / rp);
     if (mode == 0) {
-        // plain: SpMM kernel, then a vector pass, then a dot kernel per step
-        const int64_t rp = std::max<int64_t>(256, (N + G - 1) / G);
-        const int ndot = int((N + rp - 1) / rp);
         VBH_REQUIRE(ndot <= G, "internal: dot partial blocks exceed work-groups");
         VBH_CHECK_HIP(hipMemsetAsync(partials, 0, sizeof(double) * nsteps * G * E, s));
+    }
+    StepTimer timer(s);
+    timer.start();
+    if (mode == 0) {
+        // plain: SpMM kernel, then a vector pass, then a dot kernel per step
         for (int64_t st = 0; st < nsteps; ++st) {
             T* X = t[st % 3];
             T* Y = t[(st + 1) % 3];
@@ -238,6 +270,7 @@ static void kpm_single(vbh_matrix* m, int K, const S* V0, int ldv, int M, int mo
             launch_rows<T, true>(m, a, s);
         }
     }
+    timer.stop(mode == 0 ? 3 * nsteps : nsteps);
     // one reduction over all steps
     double* red = (double*)scratch(sizeof(double) * nsteps * E, 6);
     const int hseg[2] = {0, G};
@@ -318,30 +351,70 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
         VBH_CHECK_HIP(hipStreamSynchronize(s));
         const double a1 = 1.0 / half, b1 = -center / half;
         const double a2 = 2.0 / half, b2 = -2.0 * center / half;
+        int max_block = 1;
+        for (int h = 0; h < count; ++h) max_block = std::max(max_block, ms[h]->max_block);
+        const int rb = row_config(K < 2 ? 2 : K, max_block, 0).rb;
         auto launch_all = [&](hipStream_t st) {
             for (int64_t step = 0; step < nsteps; ++step) {
-                hipLaunchKernelGGL(k_apply_batched<T>, dim3(Gtot), dim3(kThreads), 0, st, (const BatchItem<T>*)ditems,
-                                   count, Gtot, K, int(step), step == 0 ? a1 : a2, step == 0 ? b1 : b2, 0.0, -1.0, mode,
-                                   partials + step * Gtot * E);
+                const BatchItem<T>* it = (const BatchItem<T>*)ditems;
+                const double al = step == 0 ? a1 : a2, be = step == 0 ? b1 : b2;
+                double* pt = partials + step * Gtot * E;
+                switch (rb) {
+                    case 1: hipLaunchKernelGGL((k_apply_batched<T, 1>), dim3(Gtot), dim3(kThreads), 0, st, it, count,
+                                               Gtot, K, int(step), al, be, 0.0, -1.0, mode, pt); break;
+                    case 2: hipLaunchKernelGGL((k_apply_batched<T, 2>), dim3(Gtot), dim3(kThreads), 0, st, it, count,
+                                               Gtot, K, int(step), al, be, 0.0, -1.0, mode, pt); break;
+                    case 4: hipLaunchKernelGGL((k_apply_batched<T, 4>), dim3(Gtot), dim3(kThreads), 0, st, it, count,
+                                               Gtot, K, int(step), al, be, 0.0, -1.0, mode, pt); break;
+                    default: hipLaunchKernelGGL((k_apply_batched<T, 8>), dim3(Gtot), dim3(kThreads), 0, st, it, count,
+                                                Gtot, K, int(step), al, be, 0.0, -1.0, mode, pt); break;
+                }
             }
         };
         if (use_graph) {
-            hipStream_t cs;
-            VBH_CHECK_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
-            hipGraph_t graph;
-            hipGraphExec_t exec;
-            VBH_CHECK_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-            launch_all(cs);
-            VBH_CHECK_HIP(hipStreamEndCapture(cs, &graph));
-            VBH_CHECK_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-            VBH_CHECK_HIP(hipGraphLaunch(exec, cs));
-            VBH_CHECK_HIP(hipStreamSynchronize(cs));
-            VBH_CHECK_HIP(hipGraphExecDestroy(exec));
-            VBH_CHECK_HIP(hipGraphDestroy(graph));
-            VBH_CHECK_HIP(hipStreamDestroy(cs));
+            // The captured graph is kept and replayed while the batch (matrices,
+            // buffers, shape and scaling) is unchanged; a new batch re-captures.
+            static struct {
+                hipStream_t cs = nullptr;
+                hipGraph_t graph = nullptr;
+                hipGraphExec_t exec = nullptr;
+                std::vector<uintptr_t> key;
+            } cache;
+            std::vector<uintptr_t> key = {uintptr_t(ditems), uintptr_t(vecs), uintptr_t(partials), uintptr_t(count),
+                                          uintptr_t(K), uintptr_t(nsteps), uintptr_t(mode), uintptr_t(sizeof(T)),
+                                          uintptr_t(Gtot)};
+            double ab[4] = {a1, b1, a2, b2};
+            for (double v : ab) {
+                uint64_t bits;
+                std::memcpy(&bits, &v, sizeof(bits));
+                key.push_back(uintptr_t(bits));
+            }
+            for (int h = 0; h < count; ++h) {
+                key.push_back(uintptr_t(items[h].vals));
+                key.push_back(uintptr_t(ms[h]->split));
+            }
+            if (!cache.cs) VBH_CHECK_HIP(hipStreamCreateWithFlags(&cache.cs, hipStreamNonBlocking));
+            if (!cache.exec || cache.key != key) {
+                if (cache.exec) VBH_CHECK_HIP(hipGraphExecDestroy(cache.exec));
+                if (cache.graph) VBH_CHECK_HIP(hipGraphDestroy(cache.graph));
+                cache.exec = nullptr;
+                cache.graph = nullptr;
+                VBH_CHECK_HIP(hipStreamBeginCapture(cache.cs, hipStreamCaptureModeThreadLocal));
+                launch_all(cache.cs);
+                VBH_CHECK_HIP(hipStreamEndCapture(cache.cs, &cache.graph));
+                VBH_CHECK_HIP(hipGraphInstantiate(&cache.exec, cache.graph, nullptr, nullptr, 0));
+                cache.key = key;
+            }
+            StepTimer timer(cache.cs);
+            timer.start();
+            VBH_CHECK_HIP(hipGraphLaunch(cache.exec, cache.cs));
+            timer.stop(nsteps);
         } else {
+            StepTimer timer(s);
+            timer.start();
             launch_all(s);
             VBH_CHECK_HIP(hipGetLastError());
+            timer.stop(nsteps);
         }
         double* red = (double*)scratch(sizeof(double) * nsteps * count * E, 7);
         hipLaunchKernelGGL(k_reduce_segments, dim3(E, count, unsigned(nsteps)), dim3(256), 0, s, partials, Gtot, E,
@@ -472,6 +545,14 @@ VBH_API int vbh_kpm_moments_batched(int count, const vbh_matrix_t* ms, int K, co
             using S = std::remove_pointer_t<decltype(sty)>;
             kpm_batched<T, S>(count, ms, K, V0s, ldv, M, mode, center, half_width, graph, moments, (hipStream_t)stream);
         });
+    });
+}
+
+VBH_API int vbh_kpm_last_step_time(double* ms, int64_t* launches) {
+    return guard([&] {
+        VBH_REQUIRE(ms && launches, "null argument");
+        *ms = g_last_step_ms;
+        *launches = g_last_launches;
     });
 }
 

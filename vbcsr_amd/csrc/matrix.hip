@@ -224,17 +224,23 @@ vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const
         for (int j = 0; j < nbc; ++j) coff[j + 1] = coff[j] + cbs[j];
         m->n_rows = roff[nbr];
         m->n_cols = coff[nbc];
+        VBH_REQUIRE(mx <= 255, "block sizes above 255 are not supported");
         m->h_voff.assign(m->nnzb + 1, 0);
+        std::vector<int2> meta(m->nnzb);
         std::vector<int64_t> work(nbr + 1, 0);
         std::vector<char> cls(size_t(mx + 1) * (mx + 1), 0);
         for (int i = 0; i < nbr; ++i) {
             int64_t w = 64;
+            int panel = 0;
             for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
                 const int j = col_ind[k];
                 VBH_REQUIRE(j >= 0 && j < nbc, "column index out of range");
                 if (k > row_ptr[i]) VBH_REQUIRE(col_ind[k - 1] < j, "columns must be strictly ascending per row");
                 const int64_t sz = int64_t(rbs[i]) * cbs[j];
                 m->h_voff[k + 1] = m->h_voff[k] + sz;
+                meta[k] = make_int2(coff[j], (panel << 8) | cbs[j]);
+                panel += cbs[j];
+                VBH_REQUIRE(panel < (1 << 23), "block row wider than 2^23 scalar columns");
                 w += sz + 16;
                 cls[size_t(rbs[i]) * (mx + 1) + cbs[j]] = 1;
             }
@@ -263,9 +269,18 @@ vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const
         m->roff = upload(roff, s);
         m->coff = upload(coff, s);
         m->voff = upload(m->h_voff, s);
+        m->meta = upload(meta, s);
+        {
+            std::vector<int> row_of(std::max<int64_t>(m->n_rows, 1), 0);
+            for (int i = 0; i < nbr; ++i)
+                for (int t = roff[i]; t < roff[i + 1]; ++t) row_of[t] = i;
+            m->row_of = upload(row_of, s);
+        }
         m->split = upload(split, s);
-        m->vals = dalloc(m->elem_size() * std::max<int64_t>(m->nnz, 1));
-        VBH_CHECK_HIP(hipMemsetAsync(m->vals, 0, m->elem_size() * std::max<int64_t>(m->nnz, 1), s));
+        // kVecPad trailing elements: the streaming kernels read whole 16-byte
+        // words, which may reach up to one word past the last value
+        m->vals = dalloc(m->elem_size() * (m->nnz + kVecPad));
+        VBH_CHECK_HIP(hipMemsetAsync(m->vals, 0, m->elem_size() * (m->nnz + kVecPad), s));
         VBH_CHECK_HIP(hipStreamSynchronize(s));
     } catch (...) {
         destroy_matrix(m);
@@ -277,7 +292,8 @@ vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const
 void destroy_matrix(vbh_matrix* m) {
     if (!m) return;
     for (void* p : {(void*)m->rbs, (void*)m->cbs, (void*)m->row_ptr, (void*)m->col_ind, (void*)m->roff,
-                    (void*)m->coff, (void*)m->voff, m->vals, m->vals32, (void*)m->split, (void*)m->t_ptr,
+                    (void*)m->coff, (void*)m->voff, (void*)m->meta, (void*)m->row_of, m->vals, m->vals32,
+                    (void*)m->split, (void*)m->t_ptr,
                     (void*)m->t_row, (void*)m->t_slot, (void*)m->t_split})
         dfree(p);
     delete m;
@@ -285,7 +301,10 @@ void destroy_matrix(vbh_matrix* m) {
 
 void ensure_vals32(vbh_matrix* m, hipStream_t s) {
     if (m->vals32_valid) return;
-    if (!m->vals32) m->vals32 = dalloc((m->elem_size() / 2) * std::max<int64_t>(m->nnz, 1));
+    if (!m->vals32) {
+        m->vals32 = dalloc((m->elem_size() / 2) * (m->nnz + kVecPad));
+        VBH_CHECK_HIP(hipMemsetAsync(m->vals32, 0, (m->elem_size() / 2) * (m->nnz + kVecPad), s));
+    }
     const int64_t n = m->nnz;
     if (n > 0) {
         if (m->dtype == VBH_F64)

@@ -949,6 +949,13 @@ PYBIND11_MODULE(_core, m) {
         return n;
     });
     m.def("library_version", &vbh_version);
+    m.def("kpm_last_step_time", []() {
+        double ms = 0.0;
+        int64_t n = 0;
+        check(vbh_kpm_last_step_time(&ms, &n), "kpm_last_step_time");
+        return py::make_tuple(ms, n);
+    });
+    m.def("synchronize", []() { check(vbh_stream_sync(nullptr), "synchronize"); });
 
     py::enum_<AssemblyMode>(m, "AssemblyMode")
         .value("INSERT", AssemblyMode::INSERT)

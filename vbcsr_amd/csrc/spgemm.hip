@@ -69,7 +69,9 @@ struct SymArgs {
 
 template <bool LDS>
 __global__ __launch_bounds__(256) void k_symbolic(SymArgs a) {
-    extern __shared__ char smem[];
+    // double-typed so the dynamic area is 8-byte aligned: the FP64 values are
+    // updated with 8-byte LDS atomics, which require natural alignment
+    extern __shared__ double smem[];
     const int q = blockIdx.x;
     if (q >= a.nrows) return;
     const int i = a.rows[q];
@@ -77,8 +79,8 @@ __global__ __launch_bounds__(256) void k_symbolic(SymArgs a) {
     int* keys;
     double* vals;
     if constexpr (LDS) {
-        vals = reinterpret_cast<double*>(smem);
-        keys = reinterpret_cast<int*>(smem + sizeof(double) * S);
+        vals = smem;
+        keys = reinterpret_cast<int*>(smem + S);
     } else {
         keys = a.g_keys + a.g_off[q];
         vals = a.g_vals + a.g_off[q];

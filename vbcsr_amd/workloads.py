@@ -360,3 +360,43 @@ def wavepacket(nx: int, ny: int, orbitals: int, width: float, seed: int) -> np.n
     phase = np.exp(2j * np.pi * g.random(env.shape[0]))
     psi = env * phase
     return psi / np.linalg.norm(psi)
+
+
+def random_vbcsr(nb: int, bs_lo: int = 4, bs_hi: int = 8, per_row: int = 40, seed: int = 0) -> BlockCSR:
+    """Large variable-block matrix for bandwidth runs (fully vectorised):
+    block sizes U{bs_lo..bs_hi}, the diagonal block plus ``per_row`` random
+    distinct columns per row, entries U[-1, 1].  Not symmetric."""
+    rng = np.random.default_rng(seed)
+    bs = rng.integers(bs_lo, bs_hi + 1, size=nb).astype(np.int32)
+    rows = np.repeat(np.arange(nb, dtype=np.int64), per_row + 1)
+    cols = rng.integers(0, nb, size=(nb, per_row + 1)).astype(np.int64)
+    cols[:, 0] = np.arange(nb)
+    key = np.unique(rows * nb + cols.ravel())
+    r, c = key // nb, key % nb
+    row_ptr = np.zeros(nb + 1, np.int32)
+    np.add.at(row_ptr, r + 1, 1)
+    np.cumsum(row_ptr, out=row_ptr)
+    mat = BlockCSR(bs, row_ptr, c.astype(np.int32), np.zeros(0))
+    nnz = int(mat.value_offsets()[-1])
+    mat.vals = rng.uniform(-1.0, 1.0, size=nnz)
+    mat.meta.update(name="random_vbcsr", nb=nb, per_row=per_row, seed=seed)
+    return mat
+
+
+def gershgorin_scale(mat: BlockCSR, edge: float = 0.99) -> BlockCSR:
+    """Scale by the Gershgorin bound max_i sum_j |A_ij| so the spectrum lies in
+    [-edge, edge] (cheap, no eigen-solve)."""
+    rows = np.repeat(np.arange(mat.nb), np.diff(mat.row_ptr))
+    r = mat.bs[rows].astype(np.int64)
+    c = mat.bs[mat.col_ind].astype(np.int64)
+    voff = mat.value_offsets()
+    blk = np.repeat(np.arange(mat.nnzb), r * c)
+    e = np.arange(mat.nnz) - voff[blk]
+    off = mat.offsets()
+    srow = off[rows][blk] + e // c[blk]
+    rowsum = np.zeros(mat.n)
+    np.add.at(rowsum, srow, np.abs(mat.vals))
+    bound = float(rowsum.max()) if mat.n else 1.0
+    out = mat.scaled(edge / bound)
+    out.meta.update(half_width=bound / edge, center=0.0)
+    return out
