@@ -235,29 +235,37 @@ def main():
         sync()
         res = {}
 
-        def batched_fp32():
-            res["mu32"] = KP.kpm_moments_batched(mats, V0s, Mb, mode="two", precision="fp32", graph=True)
+        def batched(prec):
+            def run():
+                res[prec] = KP.kpm_moments_batched(mats, V0s, Mb, mode="two", precision=prec, graph=True)
+            return run
 
         def plain_fp64():
-            res["mu64"] = np.stack([KP.kpm_moments(Hm, v, Mb, mode="plain") for Hm, v in zip(mats, V0s)])
+            res["fp64"] = np.stack([KP.kpm_moments(Hm, v, Mb, mode="plain") for Hm, v in zip(mats, V0s)])
 
         bsteps = max(2, args.steps // 5)
-        t32 = timed(batched_fp32, bsteps, 1, sync)
+        tb = {}
+        for prec in ("mixed", "fp32"):
+            tb[prec] = timed(batched(prec), bsteps, 1, sync)
         t64 = timed(plain_fp64, max(1, bsteps // 2), 1, sync)
-        errs = []
-        for h in range(count):
-            _, r64 = KP.dos_from_moments(res["mu64"][h])
-            _, r32 = KP.dos_from_moments(res["mu32"][h])
-            errs.append(KP.dos_error(r32, r64))
+        errs = {}
+        for prec in ("mixed", "fp32"):
+            e = []
+            for h in range(count):
+                _, r64 = KP.dos_from_moments(res["fp64"][h])
+                _, rp = KP.dos_from_moments(res[prec][h])
+                e.append(KP.dos_error(rp, r64))
+            errs[prec] = float(max(e))
         rows = int(sum(A.n for A in Hs))
         out["batched"] = {"workload": f"{count} random-geometric Hamiltonians, blocks {{1,4,9,13}}, "
                                       f"{rows} rows total, {Mb} moments x {Kb} vectors each",
-                          "fp32_batched_graph_moment_vecs_per_s": count * Mb * Kb / t32,
-                          "fp64_plain_moment_vecs_per_s": count * Mb * Kb / t64,
-                          "speedup": t64 / t32, "max_dos_error": float(max(errs)),
-                          "nnz_total": int(sum(A.nnz for A in Hs))}
-        log(f"batched: fp32 graph {t32 * 1e3:.2f} ms, fp64 plain {t64 * 1e3:.2f} ms, speedup {t64 / t32:.2f}x, "
-            f"max DOS err {max(errs):.2e}")
+                          "nnz_total": int(sum(A.nnz for A in Hs)),
+                          "fp64_plain_moment_vecs_per_s": count * Mb * Kb / t64}
+        for prec in ("mixed", "fp32"):
+            out["batched"][prec] = {"moment_vecs_per_s": count * Mb * Kb / tb[prec],
+                                    "speedup_vs_fp64_plain": t64 / tb[prec], "max_dos_error": errs[prec]}
+            log(f"batched {prec}: {tb[prec] * 1e3:.2f} ms, speedup {t64 / tb[prec]:.2f}x vs fp64 plain "
+                f"{t64 * 1e3:.2f} ms, max DOS err {errs[prec]:.2e}")
 
     cpu = None
     if want("cpu") and not args.no_cpu and not args.quick:

@@ -96,10 +96,16 @@ VBH_API int vbh_matrix_duplicate(vbh_matrix_t m, vbh_matrix_t* out, void* stream
 
 /* ---- applies ------------------------------------------------------------
  * precision: 0 = the matrix's own type (FP64 / complex128 vectors),
- *            1 = FP32 mixed (float / complex<float> vectors; the matrix values
- *                are read from an FP32 copy kept on the device).
+ *            1 = FP32 (float / complex<float> vectors; the matrix values
+ *                are read from an FP32 copy kept on the device),
+ *            2 = FP32 values against the matrix's own vector type (FP64 /
+ *                complex128 vectors, FP32 value copy): the value stream
+ *                halves, the vectors keep full precision.  1 <= K <= 64;
+ *                not for the adjoint or per-column fused dots.
+ * Every precision accumulates in FP64 registers.
  * x has n_cols entries (owned + ghost columns), y has n_rows; y is fully
- * overwritten.  Multivectors: K columns, leading dims ldx, ldy >= K. */
+ * overwritten.  Multivectors: K columns, leading dims ldx, ldy >= K; only
+ * columns < K of each row are read or written. */
 /* BlockSpMat::mult, block_csr.hpp:985 (vbcsr_apply.hpp:25-42, bsr_apply.hpp:331-408). */
 VBH_API int vbh_spmv(vbh_matrix_t m, const void* x, void* y, int precision, void* stream);
 /* BlockSpMat::mult_dense, block_csr.hpp:1011 (vbcsr_apply.hpp:277-340, bsr_apply.hpp:410-485). */
@@ -126,7 +132,9 @@ VBH_API int vbh_spmm_fused(vbh_matrix_t m, int K, const void* X, int ldx, void* 
  * the matrix's type (square block structure, no ghosts).  mode 1: one moment
  * per matvec (examples/ex08_graphene_dos.py:95-114); mode 2: two moments per
  * matvec (examples/inp_nanocrystal_dos/kpm.py:46-73).  precision 1 runs the
- * recursion in FP32 with FP64 moment reductions.  moments: host, M doubles. */
+ * recursion in FP32 (FP32 vectors and values), precision 2 with FP32 values
+ * and FP64 vectors; the moment reductions are FP64 in every precision.
+ * moments: host, M doubles. */
 VBH_API int vbh_kpm_moments(vbh_matrix_t m, int K, const void* V0, int ldv, int M, int mode,
                             double center, double half_width, int precision, double* moments,
                             void* stream);

@@ -105,6 +105,45 @@ def test_batched_matches_single(graph):
         assert relerr(mu32[h], ref) < 1e-5
 
 
+def rounded32(A):
+    """The same matrix with its values rounded to FP32 (the value copy that
+    precision "mixed" reads), kept in FP64 for the oracle."""
+    from vbcsr_amd.workloads import BlockCSR
+
+    v = A.vals
+    if np.iscomplexobj(v):
+        v32 = v.real.astype(np.float32).astype(np.float64) + 1j * v.imag.astype(np.float32).astype(np.float64)
+    else:
+        v32 = v.astype(np.float32).astype(np.float64)
+    return BlockCSR(A.bs, A.row_ptr, A.col_ind, v32)
+
+
+@pytest.mark.parametrize("K", [1, 7, 32])
+def test_mixed_precision_single_and_batched(K):
+    """precision "mixed": FP32 values against FP64 vectors -- equal, to FP64
+    rounding, to the FP64 recursion on the FP32-rounded matrix."""
+    from vbcsr_amd import kpm as KP
+    from vbcsr_amd import workloads as W
+
+    Hs = W.batched_hamiltonians(count=4, nb_lo=40, nb_hi=200, neighbours=8.0, seed=5)
+    mats = [mk(A) for A in Hs]
+    V0s = []
+    for h, H in enumerate(mats):
+        v = H.create_multivector(K)
+        v.set_random(200 + h, normalize=False)
+        V0s.append(v)
+    refs = [O.kpm_moments(O.Mat.of(rounded32(A)), np.asarray(v.to_numpy()).reshape(A.n, K), 40, mode=1)
+            for A, v in zip(Hs, V0s)]
+    for mode in ("plain", "one", "two"):
+        for h in range(len(mats)):
+            mu = KP.kpm_moments(mats[h], V0s[h], 40, mode=mode, precision="mixed")
+            assert relerr(mu, refs[h]) < 1e-11, (mode, h)
+    for mode in ("one", "two"):
+        mub = KP.kpm_moments_batched(mats, V0s, 40, mode=mode, precision="mixed")
+        for h in range(len(mats)):
+            assert relerr(mub[h], refs[h]) < 1e-11, (mode, h)
+
+
 def test_kpm_apply_exp_closed_form():
     """tests/test_kpm.py:7-58 through the drop-in kpm_apply."""
     from vbcsr_amd import VBCSR

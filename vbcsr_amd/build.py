@@ -18,7 +18,9 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
-HIP_SOURCES = ["matrix.hip", "apply.hip", "vector.hip", "kpm.hip", "spgemm.hip"]
+HIP_SOURCES = ["matrix.hip", "apply.hip", "vector.hip", "kpm.hip", "spgemm.hip",
+               "lanes_f64.hip", "lanes_f32.hip", "lanes_c128.hip", "lanes_c64.hip", "lanes_f64m.hip",
+               "lanes_c128m.hip"]
 LIB = os.path.join(HERE, "libvbcsr_hip.so")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
@@ -46,7 +48,7 @@ def _run(cmd):
     return r.stdout
 
 
-def build(verbose: bool = False, jobs: int = 6) -> None:
+def build(verbose: bool = False, jobs: int = 8) -> None:
     os.makedirs(OBJ, exist_ok=True)
 
     def compile_one(src):

@@ -29,7 +29,10 @@ __global__ __launch_bounds__(kThreads) void k_apply_rows(ApplyArgs<T> g) {
     double d1 = 0.0, d2 = 0.0;
     row_loop<T, BS, RB, FUSED>(g, g.split[chunk], g.split[chunk + 1], lane, wave, d1, d2);
     if constexpr (FUSED) {
-        if (g.partials != nullptr) write_partials(g.K, lane, wave, d1, d2, g.partials + int64_t(chunk) * 2 * g.K);
+        if (g.partials != nullptr) {
+            const int E = g.dot_cols > 1 ? g.K : 1;
+            write_partials(E, lane, wave, d1, d2, g.partials + int64_t(chunk) * 2 * E);
+        }
     }
 }
 
@@ -135,8 +138,29 @@ static void launch_spmm_family(int rb, const ApplyArgs<T>& a, hipStream_t s) {
     }
 }
 
+// Single-type row families: LDS SpMV, row_loop SpMM (defined below).
 template <typename T, bool FUSED>
-void launch_rows(const vbh_matrix* m, const ApplyArgs<T>& a, hipStream_t s) {
+static void launch_single(const vbh_matrix* m, const ApplyArgs<T>& a, hipStream_t s);
+
+template <typename T, bool FUSED, typename V>
+void launch_rows(const vbh_matrix* m, const ApplyArgs<T, V>& a, hipStream_t s) {
+    if (uses_lanes(a)) {
+        const LaneShape sh = lane_shape_of<T, V>(a.K);
+        ApplyArgs<T, V> b = a;
+        b.G = lane_groups(m->n_rows, sh.S);
+        launch_lanes<T, V>(b, m->n_rows, sh, m->uniform_bs == 4 ? 4 : 0, s);
+        return;
+    }
+    if constexpr (is_mixed<T, V>()) {
+        throw Error(VBH_ERR_NOTSUP,
+                    "precision 2 (FP32 values, native vectors) supports 1 <= K <= 64 and column-summed dots only");
+    } else {
+        launch_single<T, FUSED>(m, a, s);
+    }
+}
+
+template <typename T, bool FUSED>
+static void launch_single(const vbh_matrix* m, const ApplyArgs<T>& a, hipStream_t s) {
     const RowConfig c = row_config(a.K, m->max_block, m->uniform_bs);
     const dim3 grid(a.G), block(kThreads);
     if (c.spmv_rpp > 0) {
@@ -154,9 +178,9 @@ void launch_rows(const vbh_matrix* m, const ApplyArgs<T>& a, hipStream_t s) {
     VBH_CHECK_HIP(hipGetLastError());
 }
 
-template <typename T>
-ApplyArgs<T> make_args(const vbh_matrix* m, const T* vals, int K, const T* X, int64_t ldx, T* Y, int64_t ldy) {
-    ApplyArgs<T> a{};
+template <typename T, typename V>
+ApplyArgs<T, V> make_args(const vbh_matrix* m, const V* vals, int K, const T* X, int64_t ldx, T* Y, int64_t ldy) {
+    ApplyArgs<T, V> a{};
     a.p = m->pattern();
     a.vals = vals;
     a.K = K;
@@ -170,11 +194,11 @@ ApplyArgs<T> make_args(const vbh_matrix* m, const T* vals, int K, const T* X, in
     return a;
 }
 
-template <typename T>
-void apply_plain(vbh_matrix* m, const T* vals, int K, const void* X, int64_t ldx, void* Y, int64_t ldy, hipStream_t s) {
+template <typename T, typename V>
+void apply_plain(vbh_matrix* m, const V* vals, int K, const void* X, int64_t ldx, void* Y, int64_t ldy, hipStream_t s) {
     if (m->nbr == 0 || K == 0) return;
-    auto a = make_args<T>(m, vals, K, (const T*)X, ldx, (T*)Y, ldy);
-    launch_rows<T, false>(m, a, s);
+    auto a = make_args<T, V>(m, vals, K, (const T*)X, ldx, (T*)Y, ldy);
+    launch_rows<T, false, V>(m, a, s);
 }
 
 template <typename T>
@@ -215,18 +239,23 @@ const T* compute_values(vbh_matrix* m, hipStream_t s) {
 }
 
 // explicit instantiations used by kpm.hip
-#define VBH_INST(T)                                                                                             \
-    template void launch_rows<T, true>(const vbh_matrix*, const ApplyArgs<T>&, hipStream_t);                   \
-    template void launch_rows<T, false>(const vbh_matrix*, const ApplyArgs<T>&, hipStream_t);                  \
-    template ApplyArgs<T> make_args<T>(const vbh_matrix*, const T*, int, const T*, int64_t, T*, int64_t);      \
-    template const T* compute_values<T>(vbh_matrix*, hipStream_t);                                             \
-    template void apply_plain<T>(vbh_matrix*, const T*, int, const void*, int64_t, void*, int64_t, hipStream_t); \
+#define VBH_INST2(T, V)                                                                                          \
+    template void launch_rows<T, true, V>(const vbh_matrix*, const ApplyArgs<T, V>&, hipStream_t);              \
+    template void launch_rows<T, false, V>(const vbh_matrix*, const ApplyArgs<T, V>&, hipStream_t);             \
+    template ApplyArgs<T, V> make_args<T, V>(const vbh_matrix*, const V*, int, const T*, int64_t, T*, int64_t); \
+    template void apply_plain<T, V>(vbh_matrix*, const V*, int, const void*, int64_t, void*, int64_t, hipStream_t);
+#define VBH_INST(T)                                                          \
+    VBH_INST2(T, T)                                                          \
+    template const T* compute_values<T>(vbh_matrix*, hipStream_t);          \
     template void apply_adjoint<T>(vbh_matrix*, const T*, int, const void*, int64_t, void*, int64_t, hipStream_t);
 VBH_INST(double)
 VBH_INST(float)
 VBH_INST(c128)
 VBH_INST(c64)
+VBH_INST2(double, float)
+VBH_INST2(c128, c64)
 #undef VBH_INST
+#undef VBH_INST2
 
 }  // namespace vbh
 
@@ -239,9 +268,10 @@ VBH_API int vbh_spmm(vbh_matrix_t m, int K, const void* X, int ldx, void* Y, int
         VBH_REQUIRE(m, "null matrix");
         VBH_REQUIRE(K >= 0 && ldx >= K && ldy >= K, "leading dimensions must be >= K");
         hipStream_t s = (hipStream_t)stream;
-        dispatch_compute(m, precision, [&](auto* tag) {
+        dispatch_compute(m, precision, [&](auto* tag, auto* vtag) {
             using T = std::remove_pointer_t<decltype(tag)>;
-            apply_plain<T>(m, compute_values<T>(m, s), K, X, ldx, Y, ldy, s);
+            using V = std::remove_pointer_t<decltype(vtag)>;
+            apply_plain<T, V>(m, compute_values<V>(m, s), K, X, ldx, Y, ldy, s);
         });
     });
 }
@@ -256,7 +286,8 @@ VBH_API int vbh_spmm_adjoint(vbh_matrix_t m, int K, const void* X, int ldx, void
         VBH_REQUIRE(m, "null matrix");
         VBH_REQUIRE(K >= 0 && ldx >= K && ldy >= K, "leading dimensions must be >= K");
         hipStream_t s = (hipStream_t)stream;
-        dispatch_compute(m, precision, [&](auto* tag) {
+        VBH_REQUIRE(precision != 2, "the adjoint supports precision 0 and 1");
+        dispatch_compute(m, precision, [&](auto* tag, auto*) {
             using T = std::remove_pointer_t<decltype(tag)>;
             apply_adjoint<T>(m, compute_values<T>(m, s), K, X, ldx, Y, ldy, s);
         });
@@ -280,9 +311,11 @@ VBH_API int vbh_spmm_fused(vbh_matrix_t m, int K, const void* X, int ldx, void* 
         VBH_REQUIRE(m->nbc >= m->nbr, "fused apply needs the owned columns of a square pattern");
         if (m->nbr == 0 || K == 0) return;
         hipStream_t s = (hipStream_t)stream;
-        dispatch_compute(m, precision, [&](auto* tag) {
+        VBH_REQUIRE(precision != 2 || !dots, "per-column dots support precision 0 and 1");
+        dispatch_compute(m, precision, [&](auto* tag, auto* vtag) {
             using T = std::remove_pointer_t<decltype(tag)>;
-            auto a = make_args<T>(m, compute_values<T>(m, s), K, (const T*)X, ldx, (T*)Y, ldy);
+            using V = std::remove_pointer_t<decltype(vtag)>;
+            auto a = make_args<T, V>(m, compute_values<V>(m, s), K, (const T*)X, ldx, (T*)Y, ldy);
             a.alpha = alpha;
             a.beta = beta;
             a.gamma = gamma;
@@ -290,11 +323,13 @@ VBH_API int vbh_spmm_fused(vbh_matrix_t m, int K, const void* X, int ldx, void* 
             a.ldz = ldz;
             a.W = (const T*)W;
             a.ldw = ldw;
+            a.dot_cols = dots ? K : 0;
+            const int G = apply_groups(m, a);
             double* part = nullptr;
-            if (dots) part = (double*)scratch(sizeof(double) * 2 * K * m->G, 2);
+            if (dots) part = (double*)scratch(sizeof(double) * 2 * K * G, 2);
             a.partials = part;
-            launch_rows<T, true>(m, a, s);
-            if (dots) reduce_partials(part, m->G, 2 * K, dots, true, s);
+            launch_rows<T, true, V>(m, a, s);
+            if (dots) reduce_partials(part, G, 2 * K, dots, true, s);
         });
     });
 }

@@ -20,14 +20,20 @@
 //   against x gathered from global memory.
 #pragma once
 
+#include <initializer_list>
+#include <type_traits>
+#include <utility>
+
 #include "common.hpp"
 
 namespace vbh {
 
-template <typename T>
+// T: vector (and accumulation input) type; V: matrix value type, T or its
+// FP32 partner (precision 2: FP32 values against FP64 vectors).
+template <typename T, typename V = T>
 struct ApplyArgs {
     DevPattern p;
-    const T* vals;
+    const V* vals;
     int K;
     const T* X;
     int64_t ldx;
@@ -39,7 +45,8 @@ struct ApplyArgs {
     int64_t ldz;
     const T* W;
     int64_t ldw;
-    double* partials;  // [G][2K] per-work-group FP64 partial dots, or null
+    double* partials;  // [G][2*max(dot_cols,1)] per-work-group FP64 partial dots, or null
+    int dot_cols;      // K: one partial per column; 1 (or 0, no dots): summed over the columns
     const int* split;  // G+1 row chunk boundaries
     int G;
 };
@@ -257,18 +264,47 @@ __device__ inline void row_loop_spmv(const ApplyArgs<T>& g, int r0, int r1, int 
                 const int nvec = (lead + r * (pc_end - pc_first) + EV - 1) / EV;
                 const Vec16<T>* src = reinterpret_cast<const Vec16<T>*>(g.vals + sa);
                 Vec16<T>* dst = reinterpret_cast<Vec16<T>*>(lds);
-                for (int e = lane; e < nvec; e += 64) dst[e] = src[e];
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
                 const int u = ub + p_l;
                 const int xo = __shfl(mx, u, 64);
                 const int w = __shfl(mw, u, 64);
-                if (u <= ulast && a_l < r) {
-                    const int c = w & 255;
+                const bool act = u <= ulast && a_l < r;
+                const int c = w & 255;
+                // every load of the run is issued before any is consumed: the
+                // run's values (NV 16-byte words per lane) and this lane's x
+                // entries (c <= RPP), so the wave pays one memory latency per run
+                // Indices are clamped into range instead of guarded: a guarded
+                // load compiles to a branch with its own wait, which would
+                // serialise the run again.
+                constexpr int NV = (kSpmvTile(RPP) / EV + 63) / 64;
+                Vec16<T> stage[NV];
+#pragma unroll
+                for (int q = 0; q < NV; ++q) {
+                    const int e = q * 64 + lane;
+                    stage[q] = src[e < nvec ? e : nvec - 1];
+                }
+                T xv[RPP];
+                const T* xb = g.X + int64_t(xo) * g.ldx;  // xo is a valid row for every lane
+                const int cm = c > 0 ? c - 1 : 0;
+#pragma unroll
+                for (int b = 0; b < RPP; ++b) xv[b] = xb[int64_t(b < cm ? b : cm) * g.ldx];
+                // unconditional stores too (a guarded store lets the compiler sink
+                // its load into the branch): lanes past the run rewrite word
+                // nvec-1 with the value it already holds
+#pragma unroll
+                for (int q = 0; q < NV; ++q) {
+                    const int e = q * 64 + lane;
+                    dst[e < nvec ? e : nvec - 1] = stage[q];
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                if (act) {
                     const T* ab = lds + lead + r * ((w >> 8) - pc_first) + a_l * c;
-                    const T* xb = g.X + int64_t(xo) * g.ldx;
-                    for (int b = 0; b < c; ++b) acc = Sc<A>::fma(widen(ab[b]), widen(xb[int64_t(b) * g.ldx]), acc);
+#pragma unroll
+                    for (int b = 0; b < RPP; ++b) {
+                        const A t = Sc<A>::fma(widen(ab[b < cm ? b : cm]), widen(xv[b]), acc);
+                        acc = b < c ? t : acc;
+                    }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
@@ -314,29 +350,118 @@ struct RowConfig {
 };
 RowConfig row_config(int K, int max_block, int uniform_bs);
 
-template <typename T, bool FUSED>
-void launch_rows(const vbh_matrix* m, const ApplyArgs<T>& a, hipStream_t s);
+// Lane-per-row family (lanes.hpp): S lanes per scalar row, KT columns each.
+struct LaneShape {
+    int S, KT;  // S == 0: family not applicable
+};
+// Instantiated shapes: real K <= 16 one lane per row (KT in 4, 8, 10, 12, 16),
+// up to 64 with 2 or 4 lanes of 16 columns; complex KT = 8 with 1..8 lanes.
+// K == 1 runs here only for mixed value/vector types (the SpMV families are
+// single-type).
+inline LaneShape lane_shape(bool complex, int K, bool mixed = false) {
+    if (K == 1 && mixed) K = 2;
+    if (K < 2 || K > 64) return {0, 0};
+    if (complex) {
+        if (K <= 4) return {1, 4};
+        if (K <= 8) return {1, 8};
+        return {K <= 16 ? 2 : K <= 32 ? 4 : 8, 8};
+    }
+    if (K <= 16) return {1, K <= 4 ? 4 : K <= 8 ? 8 : K <= 10 ? 10 : K <= 12 ? 12 : 16};
+    return {K <= 32 ? 2 : 4, 16};
+}
+// Scalar rows handled by one work-group of the lane family.
+inline int lane_rows_per_group(int S) { return kWaves * (64 / S); }
+// Work-groups of the lane family over n rows (a multiple of 8 for the XCD map).
+inline int lane_groups(int64_t n_rows, int S) {
+    const int64_t rpg = lane_rows_per_group(S);
+    const int64_t g = (n_rows + rpg - 1) / rpg;
+    return int(((g < 8 ? 8 : g) + 7) & ~int64_t(7));
+}
+// Widest vector (elements of T, at most 16 bytes, dividing KT and K, so no
+// vector straddles column K) that every operand (base pointer, leading
+// dimension) admits; null operands are skipped.
 template <typename T>
-ApplyArgs<T> make_args(const vbh_matrix* m, const T* vals, int K, const T* X, int64_t ldx, T* Y, int64_t ldy);
+int lane_vector(int KT, int K, std::initializer_list<std::pair<const void*, int64_t>> operands) {
+    int vn = sizeof(T) >= 16 ? 1 : int(16 / sizeof(T));
+    for (; vn > 1; vn >>= 1) {
+        bool ok = KT % vn == 0 && K % vn == 0;
+        for (const auto& o : operands)
+            if (o.first && (reinterpret_cast<uintptr_t>(o.first) % (vn * sizeof(T)) != 0 || o.second % vn != 0))
+                ok = false;
+        if (ok) break;
+    }
+    return vn;
+}
+
+template <typename T, typename V>
+constexpr bool is_mixed() {
+    return !std::is_same_v<T, V>;
+}
+template <typename T, typename V>
+inline LaneShape lane_shape_of(int K) {
+    return lane_shape(Sc<T>::is_complex, K, is_mixed<T, V>());
+}
+// Whether launch_rows runs the lane family for these arguments (per-column
+// dots stay with row_loop; mixed value/vector types exist only as lanes).
+template <typename T, typename V>
+inline bool uses_lanes(const ApplyArgs<T, V>& a) {
+    return lane_shape_of<T, V>(a.K).S > 0 && a.dot_cols <= 1;
+}
+// Work-groups (= partial slots) a launch_rows call with these arguments uses.
+template <typename T, typename V>
+inline int apply_groups(const vbh_matrix* m, const ApplyArgs<T, V>& a) {
+    return uses_lanes(a) ? lane_groups(m->n_rows, lane_shape_of<T, V>(a.K).S) : m->G;
+}
+
+template <typename T, bool FUSED, typename V = T>
+void launch_rows(const vbh_matrix* m, const ApplyArgs<T, V>& a, hipStream_t s);
+template <typename T, typename V>
+void launch_lanes(const ApplyArgs<T, V>& a, int64_t n_rows, LaneShape sh, int bs, hipStream_t s);
+
+// One matrix of a batched KPM launch.
+template <typename T, typename V = T>
+struct BatchItem {
+    DevPattern p;
+    const V* vals;
+    T* buf[3];     // recursion vectors t_{n-1}, t_n, t_{n+1} rotate through these
+    const T* v0;   // mode 1: the start vectors (W of the dot)
+    const int* split;
+    int64_t n_rows;
+    int G;         // this matrix's work-groups
+    int g_base;    // first global work-group
+};
+template <typename T, typename V>
+void launch_batched_lanes(const BatchItem<T, V>* items, int nitems, int Gtot, int K, int ld, LaneShape sh, int step,
+                          double alpha, double beta, double gamma0, double gamma, int mode, double* partials,
+                          hipStream_t s);
+template <typename T, typename V = T>
+ApplyArgs<T, V> make_args(const vbh_matrix* m, const V* vals, int K, const T* X, int64_t ldx, T* Y, int64_t ldy);
 template <typename T>
 const T* compute_values(vbh_matrix* m, hipStream_t s);
-template <typename T>
-void apply_plain(vbh_matrix* m, const T* vals, int K, const void* X, int64_t ldx, void* Y, int64_t ldy, hipStream_t s);
+template <typename T, typename V = T>
+void apply_plain(vbh_matrix* m, const V* vals, int K, const void* X, int64_t ldx, void* Y, int64_t ldy, hipStream_t s);
 template <typename T>
 void apply_adjoint(vbh_matrix* m, const T* vals, int K, const void* X, int64_t ldx, void* Y, int64_t ldy,
                    hipStream_t s);
 void reduce_partials(const double* partials, int G, int E, double* out, bool accumulate, hipStream_t s);
 
-// dtype/precision dispatch: calls f with a typed null pointer of the compute type.
+// dtype/precision dispatch: calls f(T*, V*) with typed null pointers of the
+// vector type T and the matrix value type V.  precision 0: native (FP64 /
+// complex128) throughout; 1: FP32 / complex64 throughout; 2: FP32 values
+// against native vectors (the value stream halves, vectors keep full
+// precision).
 template <typename F>
 void dispatch_compute(const vbh_matrix* m, int precision, F&& f) {
-    VBH_REQUIRE(precision == 0 || precision == 1, "precision must be 0 (native) or 1 (FP32)");
+    VBH_REQUIRE(precision >= 0 && precision <= 2,
+                "precision must be 0 (native), 1 (FP32) or 2 (FP32 values, native vectors)");
     if (m->dtype == VBH_F64) {
-        if (precision == 0) f((double*)nullptr);
-        else f((float*)nullptr);
+        if (precision == 0) f((double*)nullptr, (double*)nullptr);
+        else if (precision == 1) f((float*)nullptr, (float*)nullptr);
+        else f((double*)nullptr, (float*)nullptr);
     } else {
-        if (precision == 0) f((c128*)nullptr);
-        else f((c64*)nullptr);
+        if (precision == 0) f((c128*)nullptr, (c128*)nullptr);
+        else if (precision == 1) f((c64*)nullptr, (c64*)nullptr);
+        else f((c128*)nullptr, (c64*)nullptr);
     }
 }
 

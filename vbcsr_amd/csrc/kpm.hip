@@ -89,17 +89,6 @@ __global__ void k_reduce_segments(const double* __restrict__ partials, int Gtot,
 }
 
 // ------------------------------------------------------------ batched step
-template <typename T>
-struct BatchItem {
-    DevPattern p;
-    const T* vals;
-    T* buf[3];     // recursion vectors t_{n-1}, t_n, t_{n+1} rotate through these
-    const T* v0;   // mode 1: the start vectors (W of the dot)
-    const int* split;
-    int G;         // this matrix's work-groups
-    int g_base;    // first global work-group
-};
-
 // One launch per recursion step over every matrix of the batch: work-group g
 // (XCD-remapped) finds its matrix by binary search over g_base, then runs the
 // shared row loop on its chunk.  step >= 0 produces t_{step+1}.
@@ -135,8 +124,8 @@ __global__ __launch_bounds__(kThreads) void k_apply_batched(const BatchItem<T>* 
     a.W = mode == 1 ? it.v0 : nullptr;
     a.ldw = K;
     double d1 = 0.0, d2 = 0.0;
-    row_loop<T, 0, RB, true>(a, it.split[chunk], it.split[chunk + 1], lane, wave, d1, d2);
-    write_partials(K, lane, wave, d1, d2, partials + int64_t(gg) * 2 * K);
+    if (chunk < it.G) row_loop<T, 0, RB, true>(a, it.split[chunk], it.split[chunk + 1], lane, wave, d1, d2);
+    write_partials(1, lane, wave, d1, d2, partials + int64_t(gg) * 2);
 }
 
 // HIP-event timing of the last recursion: the step kernels only (setup,
@@ -174,7 +163,15 @@ static int64_t step_count(int M, int mode) {
     return M - 1;
 }
 
-// Assemble moments from the per-step column sums d[s][0..K) (d1) and [K..2K) (d2).
+// Bytes of one recursion buffer, rounded to 256 so that every buffer of a
+// set starts 256-byte aligned (vector loads in the step kernels).
+template <typename T>
+static size_t vec_bytes_of(int64_t N, int K) {
+    return (sizeof(T) * size_t(std::max<int64_t>(N * K, 1)) + 255) & ~size_t(255);
+}
+
+// Assemble moments from the per-step dots d[s][0..K) (d1) and [K..2K) (d2),
+// K partial columns (1 when the step kernel already summed the columns).
 static void assemble_moments(int M, int mode, int K, double mu0, const double* d, double* mu) {
     for (int n = 0; n < M; ++n) mu[n] = 0.0;
     mu[0] = mu0;
@@ -198,12 +195,12 @@ static void assemble_moments(int M, int mode, int K, double mu0, const double* d
     }
 }
 
-template <typename T, typename S>
+template <typename T, typename S, typename V>
 static void kpm_single(vbh_matrix* m, int K, const S* V0, int ldv, int M, int mode, double center, double half,
                        double* moments, hipStream_t s) {
     const int64_t N = m->n_rows;
     const int64_t nsteps = step_count<T>(M, mode);
-    const size_t vec_bytes = sizeof(T) * std::max<int64_t>(N * K, 1);
+    const size_t vec_bytes = vec_bytes_of<T>(N, K);
     const bool keep_v0 = mode != 2;  // modes 0/1 dot every t_n against v0
     T* buf = (T*)scratch(3 * vec_bytes + (keep_v0 ? vec_bytes : 0), 4);
     T* t[3] = {buf, (T*)((char*)buf + vec_bytes), (T*)((char*)buf + 2 * vec_bytes)};
@@ -228,9 +225,17 @@ static void kpm_single(vbh_matrix* m, int K, const S* V0, int ldv, int M, int mo
         if (M >= 1) moments[0] = mu0;
         return;
     }
-    const T* vals = compute_values<T>(m, s);
-    const int G = m->G;
-    const int E = 2 * K;
+    const V* vals = compute_values<V>(m, s);
+    // fused steps sum their dots over the columns (E = 2); the plain path's
+    // dot kernel keeps one partial per column (E = 2K)
+    const int Kd = mode == 0 ? K : 1;
+    const int E = 2 * Kd;
+    int G = m->G;
+    if (mode != 0) {
+        auto probe = make_args<T, V>(m, vals, K, t[0], K, t[1], K);
+        probe.dot_cols = 1;
+        G = apply_groups(m, probe);
+    }
     double* partials = (double*)scratch(sizeof(double) * nsteps * G * E, 5);
     const double a1 = 1.0 / half, b1 = -center / half;  // t1 = H~ t0
     const double a2 = 2.0 / half, b2 = -2.0 * center / half;  // t_{n+1} = 2 H~ t_n - t_{n-1}
@@ -248,7 +253,7 @@ static void kpm_single(vbh_matrix* m, int K, const S* V0, int ldv, int M, int mo
             T* X = t[st % 3];
             T* Y = t[(st + 1) % 3];
             T* Z = t[(st + 2) % 3];
-            apply_plain<T>(m, vals, K, X, K, Y, K, s);
+            apply_plain<T, V>(m, vals, K, X, K, Y, K, s);
             hipLaunchKernelGGL(k_axpbypcz<T>, dim3(nb), dim3(256), 0, s, N * K, st == 0 ? a1 : a2, Y,
                                st == 0 ? b1 : b2, (const T*)X, st == 0 ? 0.0 : -1.0, (const T*)Z);
             VBH_CHECK_HIP(hipGetLastError());
@@ -258,7 +263,7 @@ static void kpm_single(vbh_matrix* m, int K, const S* V0, int ldv, int M, int mo
         }
     } else {
         for (int64_t st = 0; st < nsteps; ++st) {
-            auto a = make_args<T>(m, vals, K, t[st % 3], K, t[(st + 1) % 3], K);
+            auto a = make_args<T, V>(m, vals, K, t[st % 3], K, t[(st + 1) % 3], K);
             a.alpha = st == 0 ? a1 : a2;
             a.beta = st == 0 ? b1 : b2;
             a.gamma = st == 0 ? 0.0 : -1.0;
@@ -266,8 +271,9 @@ static void kpm_single(vbh_matrix* m, int K, const S* V0, int ldv, int M, int mo
             a.ldz = K;
             a.W = mode == 1 ? v0 : nullptr;
             a.ldw = K;
+            a.dot_cols = 1;
             a.partials = partials + st * G * E;
-            launch_rows<T, true>(m, a, s);
+            launch_rows<T, true, V>(m, a, s);
         }
     }
     timer.stop(mode == 0 ? 3 * nsteps : nsteps);
@@ -281,53 +287,63 @@ static void kpm_single(vbh_matrix* m, int K, const S* V0, int ldv, int M, int mo
     std::vector<double> d(nsteps * E);
     VBH_CHECK_HIP(hipMemcpyAsync(d.data(), red, sizeof(double) * nsteps * E, hipMemcpyDeviceToHost, s));
     VBH_CHECK_HIP(hipStreamSynchronize(s));
-    assemble_moments(M, mode, K, mu0, d.data(), moments);
+    assemble_moments(M, mode, Kd, mu0, d.data(), moments);
 }
 
+// f(T*, S*, V*): recursion vector type T, start-vector (input) type S, matrix
+// value type V, for precision 0 (native), 1 (FP32) or 2 (FP32 values).
 template <typename F>
 static void dispatch_kpm(const vbh_matrix* m, int precision, F&& f) {
-    VBH_REQUIRE(precision == 0 || precision == 1, "precision must be 0 or 1");
+    VBH_REQUIRE(precision >= 0 && precision <= 2, "precision must be 0, 1 or 2");
     if (m->dtype == VBH_F64) {
-        if (precision == 0) f((double*)nullptr, (double*)nullptr);
-        else f((float*)nullptr, (double*)nullptr);
+        if (precision == 0) f((double*)nullptr, (double*)nullptr, (double*)nullptr);
+        else if (precision == 1) f((float*)nullptr, (double*)nullptr, (float*)nullptr);
+        else f((double*)nullptr, (double*)nullptr, (float*)nullptr);
     } else {
-        if (precision == 0) f((c128*)nullptr, (c128*)nullptr);
-        else f((c64*)nullptr, (c128*)nullptr);
+        if (precision == 0) f((c128*)nullptr, (c128*)nullptr, (c128*)nullptr);
+        else if (precision == 1) f((c64*)nullptr, (c128*)nullptr, (c64*)nullptr);
+        else f((c128*)nullptr, (c128*)nullptr, (c64*)nullptr);
     }
 }
 
 // ------------------------------------------------------------ batched driver
-template <typename T, typename S>
+template <typename T, typename S, typename V>
 static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* const* V0s, const int* ldv, int M, int mode,
                         double center, double half, int use_graph, double* moments, hipStream_t s) {
     VBH_REQUIRE(mode == 1 || mode == 2, "batched KPM supports the fused modes 1 and 2");
     VBH_REQUIRE(K >= 1 && K <= 64, "batched KPM supports 1 <= K <= 64");
     const int64_t nsteps = step_count<T>(M, mode);
-    std::vector<BatchItem<T>> items(count);
+    std::vector<BatchItem<T, V>> items(count);
     std::vector<int> seg(count + 1, 0);
     std::vector<size_t> off(count + 1, 0);
+    // K >= 2: lane family, G_h = ceil(rows / rows-per-group); K == 1: row loop
+    // over each matrix's work-balanced split.  Dots are summed over the
+    // columns (E = 2) in both.
+    const LaneShape sh = lane_shape_of<T, V>(K);
     for (int h = 0; h < count; ++h) {
         VBH_REQUIRE(ms[h]->nbc == ms[h]->nbr, "batched KPM needs square local patterns without ghosts");
-        off[h + 1] = off[h] + sizeof(T) * size_t(ms[h]->n_rows) * K * (mode == 1 ? 4 : 3);
-        seg[h + 1] = seg[h] + ms[h]->G;
+        off[h + 1] = off[h] + vec_bytes_of<T>(ms[h]->n_rows, K) * (mode == 1 ? 4 : 3);
+        const int64_t rpg = sh.S ? lane_rows_per_group(sh.S) : 1;
+        seg[h + 1] = seg[h] + (sh.S ? int((ms[h]->n_rows + rpg - 1) / rpg) : ms[h]->G);
     }
-    const int Gtot = seg[count];
-    const int E = 2 * K;
+    const int Gtot = (seg[count] + 7) & ~7;  // trailing groups find no rows
+    const int E = 2;
     char* vecs = (char*)scratch(std::max<size_t>(off[count], 64), 4);
     double* partials = (double*)scratch(sizeof(double) * std::max<int64_t>(nsteps, 1) * Gtot * E, 5);
     std::vector<double> mu0(count, 0.0);
     for (int h = 0; h < count; ++h) {
         vbh_matrix* m = ms[h];
         const int64_t N = m->n_rows;
-        const size_t vb = sizeof(T) * size_t(N) * K;
-        BatchItem<T>& it = items[h];
+        const size_t vb = vec_bytes_of<T>(N, K);
+        BatchItem<T, V>& it = items[h];
         it.p = m->pattern();
         it.p.bs = 0;
-        it.vals = compute_values<T>(m, s);
+        it.vals = compute_values<V>(m, s);
         for (int q = 0; q < 3; ++q) it.buf[q] = (T*)(vecs + off[h] + q * vb);
         it.v0 = mode == 1 ? (const T*)(vecs + off[h] + 3 * vb) : nullptr;
         it.split = m->split;
-        it.G = m->G;
+        it.n_rows = N;
+        it.G = seg[h + 1] - seg[h];
         it.g_base = seg[h];
         if (N * K > 0) {
             hipLaunchKernelGGL((k_copy_convert<S, T>), dim3(unsigned((N * K + 255) / 256)), dim3(256), 0, s, N, K,
@@ -344,9 +360,10 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
         }
     }
     if (nsteps > 0 && Gtot > 0) {
-        BatchItem<T>* ditems = (BatchItem<T>*)scratch(sizeof(BatchItem<T>) * count + sizeof(int) * (count + 1), 6);
+        using Item = BatchItem<T, V>;
+        Item* ditems = (Item*)scratch(sizeof(Item) * count + sizeof(int) * (count + 1), 6);
         int* dseg = (int*)(ditems + count);
-        VBH_CHECK_HIP(hipMemcpyAsync(ditems, items.data(), sizeof(BatchItem<T>) * count, hipMemcpyHostToDevice, s));
+        VBH_CHECK_HIP(hipMemcpyAsync(ditems, items.data(), sizeof(Item) * count, hipMemcpyHostToDevice, s));
         VBH_CHECK_HIP(hipMemcpyAsync(dseg, seg.data(), sizeof(int) * (count + 1), hipMemcpyHostToDevice, s));
         VBH_CHECK_HIP(hipStreamSynchronize(s));
         const double a1 = 1.0 / half, b1 = -center / half;
@@ -356,18 +373,24 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
         const int rb = row_config(K < 2 ? 2 : K, max_block, 0).rb;
         auto launch_all = [&](hipStream_t st) {
             for (int64_t step = 0; step < nsteps; ++step) {
-                const BatchItem<T>* it = (const BatchItem<T>*)ditems;
+                const Item* it = ditems;
                 const double al = step == 0 ? a1 : a2, be = step == 0 ? b1 : b2;
                 double* pt = partials + step * Gtot * E;
-                switch (rb) {
-                    case 1: hipLaunchKernelGGL((k_apply_batched<T, 1>), dim3(Gtot), dim3(kThreads), 0, st, it, count,
-                                               Gtot, K, int(step), al, be, 0.0, -1.0, mode, pt); break;
-                    case 2: hipLaunchKernelGGL((k_apply_batched<T, 2>), dim3(Gtot), dim3(kThreads), 0, st, it, count,
-                                               Gtot, K, int(step), al, be, 0.0, -1.0, mode, pt); break;
-                    case 4: hipLaunchKernelGGL((k_apply_batched<T, 4>), dim3(Gtot), dim3(kThreads), 0, st, it, count,
-                                               Gtot, K, int(step), al, be, 0.0, -1.0, mode, pt); break;
-                    default: hipLaunchKernelGGL((k_apply_batched<T, 8>), dim3(Gtot), dim3(kThreads), 0, st, it, count,
-                                                Gtot, K, int(step), al, be, 0.0, -1.0, mode, pt); break;
+                if (sh.S) {
+                    launch_batched_lanes<T, V>(it, count, Gtot, K, K, sh, int(step), al, be, 0.0, -1.0, mode, pt, st);
+                    continue;
+                }
+                if constexpr (!is_mixed<T, V>()) {  // K == 1, single type: row loop
+                    switch (rb) {
+                        case 1: hipLaunchKernelGGL((k_apply_batched<T, 1>), dim3(Gtot), dim3(kThreads), 0, st, it,
+                                                   count, Gtot, K, int(step), al, be, 0.0, -1.0, mode, pt); break;
+                        case 2: hipLaunchKernelGGL((k_apply_batched<T, 2>), dim3(Gtot), dim3(kThreads), 0, st, it,
+                                                   count, Gtot, K, int(step), al, be, 0.0, -1.0, mode, pt); break;
+                        case 4: hipLaunchKernelGGL((k_apply_batched<T, 4>), dim3(Gtot), dim3(kThreads), 0, st, it,
+                                                   count, Gtot, K, int(step), al, be, 0.0, -1.0, mode, pt); break;
+                        default: hipLaunchKernelGGL((k_apply_batched<T, 8>), dim3(Gtot), dim3(kThreads), 0, st, it,
+                                                    count, Gtot, K, int(step), al, be, 0.0, -1.0, mode, pt); break;
+                    }
                 }
             }
         };
@@ -427,7 +450,7 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
         for (int h = 0; h < count; ++h) {
             for (int64_t st = 0; st < nsteps; ++st)
                 std::memcpy(&dh[st * E], &d[(st * count + h) * E], sizeof(double) * E);
-            assemble_moments(M, mode, K, mu0[h], dh.data(), moments + int64_t(h) * M);
+            assemble_moments(M, mode, 1, mu0[h], dh.data(), moments + int64_t(h) * M);
         }
     } else {
         for (int h = 0; h < count; ++h) {
@@ -480,7 +503,7 @@ static void cheb_accum(int64_t N, int K, const T* t, T* Y, int64_t ldy, const do
     VBH_CHECK_HIP(hipGetLastError());
 }
 
-template <typename T>
+template <typename T, typename V>
 static void cheb_apply(vbh_matrix* m, int K, const T* X, int ldx, T* Y, int ldy, int nterms, const double* coef,
                        double center, double half, hipStream_t s) {
     const int64_t N = m->n_rows;
@@ -492,16 +515,16 @@ static void cheb_apply(vbh_matrix* m, int K, const T* X, int ldx, T* Y, int ldy,
     hipLaunchKernelGGL((k_copy_convert<T, T>), dim3(nb), dim3(256), 0, s, N, K, X, int64_t(ldx), t[0], int64_t(K));
     VBH_CHECK_HIP(hipGetLastError());
     cheb_accum<T>(N, K, t[0], Y, ldy, coef, 0, s);
-    const T* vals = compute_values<T>(m, s);
+    const V* vals = compute_values<V>(m, s);
     for (int n = 1; n < nterms; ++n) {
         const int st = n - 1;
-        auto a = make_args<T>(m, vals, K, t[st % 3], K, t[(st + 1) % 3], K);
+        auto a = make_args<T, V>(m, vals, K, t[st % 3], K, t[(st + 1) % 3], K);
         a.alpha = st == 0 ? 1.0 / half : 2.0 / half;
         a.beta = st == 0 ? -center / half : -2.0 * center / half;
         a.gamma = st == 0 ? 0.0 : -1.0;
         a.Z = t[(st + 2) % 3];
         a.ldz = K;
-        launch_rows<T, true>(m, a, s);
+        launch_rows<T, true, V>(m, a, s);
         cheb_accum<T>(N, K, t[(st + 1) % 3], Y, ldy, coef, n, s);
     }
 }
@@ -521,10 +544,11 @@ VBH_API int vbh_kpm_moments(vbh_matrix_t m, int K, const void* V0, int ldv, int 
         VBH_REQUIRE(K >= 1 && K <= 64 && ldv >= K, "KPM supports 1 <= K <= 64 columns per call, ldv >= K");
         VBH_REQUIRE(half_width > 0.0, "half_width must be positive");
         if (M <= 0) return;
-        dispatch_kpm(m, precision, [&](auto* t, auto* sty) {
+        dispatch_kpm(m, precision, [&](auto* t, auto* sty, auto* vty) {
             using T = std::remove_pointer_t<decltype(t)>;
             using S = std::remove_pointer_t<decltype(sty)>;
-            kpm_single<T, S>(m, K, (const S*)V0, ldv, M, mode, center, half_width, moments, (hipStream_t)stream);
+            using V = std::remove_pointer_t<decltype(vty)>;
+            kpm_single<T, S, V>(m, K, (const S*)V0, ldv, M, mode, center, half_width, moments, (hipStream_t)stream);
         });
     });
 }
@@ -540,10 +564,11 @@ VBH_API int vbh_kpm_moments_batched(int count, const vbh_matrix_t* ms, int K, co
             VBH_REQUIRE(ms[h]->dtype == ms[0]->dtype, "batched KPM needs one dtype");
         }
         if (M <= 0) return;
-        dispatch_kpm(ms[0], precision, [&](auto* t, auto* sty) {
+        dispatch_kpm(ms[0], precision, [&](auto* t, auto* sty, auto* vty) {
             using T = std::remove_pointer_t<decltype(t)>;
             using S = std::remove_pointer_t<decltype(sty)>;
-            kpm_batched<T, S>(count, ms, K, V0s, ldv, M, mode, center, half_width, graph, moments, (hipStream_t)stream);
+            using V = std::remove_pointer_t<decltype(vty)>;
+            kpm_batched<T, S, V>(count, ms, K, V0s, ldv, M, mode, center, half_width, graph, moments, (hipStream_t)stream);
         });
     });
 }
@@ -563,9 +588,10 @@ VBH_API int vbh_cheb_apply(vbh_matrix_t m, int K, const void* X, int ldx, void* 
         VBH_REQUIRE(m->nbc == m->nbr, "Chebyshev apply needs a square local pattern without ghosts");
         VBH_REQUIRE(ldx >= K && ldy >= K && K >= 1, "bad leading dimensions");
         VBH_REQUIRE(half_width > 0.0, "half_width must be positive");
-        dispatch_compute(m, precision, [&](auto* tag) {
+        dispatch_compute(m, precision, [&](auto* tag, auto* vtag) {
             using T = std::remove_pointer_t<decltype(tag)>;
-            cheb_apply<T>(m, K, (const T*)X, ldx, (T*)Y, ldy, nterms, coef, center, half_width, (hipStream_t)stream);
+            using V = std::remove_pointer_t<decltype(vtag)>;
+            cheb_apply<T, V>(m, K, (const T*)X, ldx, (T*)Y, ldy, nterms, coef, center, half_width, (hipStream_t)stream);
         });
     });
 }

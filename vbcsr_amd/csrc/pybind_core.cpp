@@ -837,7 +837,14 @@ void bind_mat(py::module& m, const std::string& name) {
         .def("mult_dense", [](M& self, V& x, V& y) { self.mult(x, y, 0); })
         .def("mult_adjoint", &M::mult_adjoint)
         .def("mult_dense_adjoint", &M::mult_adjoint)
-        .def("mult_precision", [](M& self, V& x, V& y, int precision) { self.mult(x, y, precision); })
+        // precision 0: FP64 values; 2: the FP32 value copy against these FP64 vectors
+        // (precision 1 needs FP32 vector buffers: spmm_raw)
+        .def("mult_precision",
+             [](M& self, V& x, V& y, int precision) {
+                 if (precision != 0 && precision != 2)
+                     throw std::invalid_argument("mult_precision: precision must be 0 or 2 for these vectors");
+                 self.mult(x, y, precision);
+             })
         .def("scale", &M::scale)
         .def("conjugate", &M::conjugate)
         .def("shift", &M::shift)
@@ -909,6 +916,8 @@ void bind_mat(py::module& m, const std::string& name) {
                                })
         .def_property_readonly("values_ptr", [](M& self) { return reinterpret_cast<uintptr_t>(vbh_matrix_values(self.dev())); })
         .def("mark_values_written", [](M& self) { self.device_written(); })
+        // the vbh_matrix_t of include/vbcsr_hip.h (values current on the device)
+        .def_property_readonly("handle", [](M& self) { return reinterpret_cast<uintptr_t>(self.dev()); })
         .def("spmm_raw",
              [](M& self, int K, uintptr_t X, int ldx, uintptr_t Y, int ldy, int precision, bool adjoint) {
                  vbh_matrix_t m = self.dev();

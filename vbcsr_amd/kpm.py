@@ -21,7 +21,8 @@ from .multivector import DistMultiVector
 from .utils import jackson_kernel
 
 _MODES = {"plain": 0, "one": 1, "two": 2}
-_PREC = {"fp64": 0, "fp32": 1}
+# fp64: FP64 throughout; fp32: FP32 vectors and values; mixed: FP32 values, FP64 vectors
+_PREC = {"fp64": 0, "fp32": 1, "mixed": 2}
 
 
 def kpm_moments(H: VBCSR, V0: DistMultiVector, num_moments: int, mode: str = "two",
@@ -31,8 +32,10 @@ def kpm_moments(H: VBCSR, V0: DistMultiVector, num_moments: int, mode: str = "tw
     mode "plain": separate SpMM / axpby / dot kernels, one moment per matvec
     (the unfused reference composition, kept as a baseline); "one": fused,
     one moment per matvec; "two": fused, two moments per matvec (needs a
-    Hermitian H).  precision "fp32" runs the recursion in FP32 with FP64 dot
-    products.  Columns beyond 64 are processed in groups of 64."""
+    Hermitian H).  precision "fp32" runs the recursion in FP32 (vectors and
+    matrix values), "mixed" with FP32 matrix values and FP64 vectors; the dot
+    products are FP64 in every precision.  Columns beyond 64 are processed in
+    groups of 64."""
     return np.asarray(H._core.kpm_moments(V0._core, int(num_moments), _MODES[mode], float(center),
                                           float(half_width), _PREC[precision]))
 

@@ -50,7 +50,7 @@ def kpm_apply(matrix, vector, coeffs, scale: float, center: float, precision: st
         res = vector.duplicate()
         res.set_constant(0.0)
         return res
-    prec = 1 if precision == "fp32" else 0
+    prec = {"fp64": 0, "fp32": 1, "mixed": 2}[precision]
     core = matrix._core.cheb_apply(vector._core, np.ascontiguousarray(coeffs, np.complex128), float(center),
                                    float(scale), prec)
     if isinstance(vector, DistMultiVector):
