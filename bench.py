@@ -48,7 +48,25 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--quick", action="store_true", help="smaller lattice (smoke of the bench itself)")
     p.add_argument("--only", default="", help="comma list: kpm,spmv,batched,cpu (default all)")
+    p.add_argument("--copy-calibration", action="store_true",
+                   help="only run a 1 GiB device-to-device copy 5 times (scale for the HBM byte counters)")
     return p.parse_args()
+
+
+def copy_calibration():
+    """A plain copy kernel of known size: 1 GiB read + 1 GiB written per launch.
+    rocprofv3's FETCH_SIZE / WRITE_SIZE read on it give the factor between the
+    counters and true HBM bytes used for the traffic figure of the roofline."""
+    import torch
+
+    n = 1 << 27  # float64 elements = 1 GiB
+    a = torch.empty(n, dtype=torch.float64, device="cuda").uniform_()
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read_per_launch": 8 * n, "copy_bytes_written_per_launch": 8 * n, "launches": 5}))
 
 
 def timed(fn, steps, warmup, sync):
@@ -116,6 +134,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; multi-GPU runs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
     only = set(filter(None, args.only.split(",")))
     want = lambda k: not only or k in only  # noqa: E731
 
@@ -167,14 +188,14 @@ def main():
         out["dos_error_fp32_vs_fp64"] = dos_err
         out["dos_error_fp64_fused_vs_fp64_plain"] = KP.dos_error(rho64f, rho64)
         out["moments_per_matvec"] = {"fp64_plain": 1, "fp64_fused": 2, "fp32_mixed": 2}
-        # roofline of the dominant kernel: the fused FP32 step (k_apply_rows<float,4,true>)
+        # roofline of the dominant kernel: the fused FP32 step (k_apply_lanes, 4x4 blocks, 10 columns)
         per_launch_ms = fp32["kernel_ms"] / max(fp32["launches"], 1)
         alg = spmm_bytes(G, K, 4, 4, with_z=True)
         achieved = alg / (per_launch_ms * 1e-3) / 1e9
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS,
                     "traffic": read_traffic("pmc_kpm_step_fp32_graphene200.txt"),
-                    "kernel": "k_apply_rows<float,4,fused> (one KPM step)",
+                    "kernel": "k_apply_lanes<float,float,4,1,10,2,true> (one fused KPM step)",
                     "alg_bytes_per_launch": alg, "avg_launch_us": per_launch_ms * 1e3,
                     "note": "working set ~95 MB fits the 256 MB Infinity Cache; see spmv_large for HBM"}
         p64 = runs["fp64_fused"]

@@ -119,7 +119,9 @@ __device__ __forceinline__ V quad_bcast(V v) {
 // every block (distinct addresses, no redundant traffic) and the quad shares
 // the 4 rows by DPP broadcast.  Column indices of 4*CB slots are preloaded
 // lane-parallel (lane a holds slots a, a+4, ...), and U slots are issued per
-// batch, so a row of 10 blocks costs 1 + 10/U memory latencies.
+// batch, so a row of 10 blocks costs 1 + 10/U memory latencies; U shrinks
+// with the slice width to keep 5+ waves per SIMD resident (measured: U = 2
+// beats 1 and 4 for 10 FP32 columns).
 template <typename T, typename V, int KT, int VN>
 __device__ __forceinline__ void quad_slots(const ApplyArgs<T, V>& g, int kb, int ke, int a, int v0, int nvalid,
                                            typename Wide<T>::type (&acc)[KT]) {
@@ -129,7 +131,7 @@ __device__ __forceinline__ void quad_slots(const ApplyArgs<T, V>& g, int kb, int
 #ifdef VBH_QUAD_BATCH
     constexpr int U = VBH_QUAD_BATCH;
 #else
-    constexpr int U = per <= 16 ? 4 : per <= 32 ? 2 : 1;    // slots per batch
+    constexpr int U = per <= 8 ? 4 : per <= 24 ? 2 : 1;     // slots per batch
 #endif
     const DevPattern& p = g.p;
     for (int k0 = kb; k0 < ke; k0 += 4 * CB) {
