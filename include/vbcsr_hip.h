@@ -155,6 +155,32 @@ VBH_API int vbh_cheb_apply(vbh_matrix_t m, int K, const void* X, int ldx, void* 
                            int nterms, const double* coef, double center, double half_width,
                            int precision, void* stream);
 
+/* ---- adjoint of the Chebyshev propagation ---------------------------------
+ * Building blocks of the reverse-mode gradient of y = sum_n c_n T_n(H~) x with
+ * respect to x and to the stored block values (no reference counterpart: the
+ * reference propagates forward only, examples/ex09_graphene_dynamics.py:102-133,
+ * vbcsr/utils.py:71-126).  Multivector layouts are row-major with the given
+ * leading dimensions; "column block n" of an n_rows x (nterms*K) array is
+ * columns [n*K, (n+1)*K).  precision 0 or 1 as for the applies. */
+/* Chebyshev basis: column block n of T receives t_n = T_n(H~) X, n < nterms. */
+VBH_API int vbh_cheb_basis(vbh_matrix_t m, int K, const void* X, int ldx, int nterms, void* T, int ldt,
+                           double center, double half_width, int precision, void* stream);
+/* Clenshaw recurrence with H~^H: column block n of B receives
+ * b_n = conj(c_n) L + 2 H~^H b_{n+1} - b_{n+2} (n >= 1) and
+ * b_0 = conj(c_0) L + H~^H b_1 - b_2, the gradient with respect to X for the
+ * cotangent L of y (torch convention: dL/dRe + i dL/dIm).  coef: nterms complex
+ * coefficients as interleaved doubles. */
+VBH_API int vbh_cheb_adjoint(vbh_matrix_t m, int K, const void* L, int ldl, int nterms, const double* coef,
+                             void* B, int ldb, double center, double half_width, int precision,
+                             void* stream);
+/* Block-sampled outer-product sum (SDDMM): for every stored block (i, j),
+ * G_blk[a][b] += alpha * sum_{q<L} U[row_i+a][q] * conj(V[col_j+b][q]).
+ * U: n_rows x L (ld ldu), V: n_cols x L (ld ldv), of the precision's vector
+ * type; G: device array of nnz values in the matrix's own type and slot order
+ * (real matrices accumulate Re).  alpha: complex as 2 doubles. */
+VBH_API int vbh_sddmm(vbh_matrix_t m, int L, const void* U, int ldu, const void* V, int ldv,
+                      const double* alpha, void* G, int precision, void* stream);
+
 /* ---- vector kernels --------------------------------------------------------
  * rows x K row-major blocks with leading dims; dtype VBH_*.  alpha/beta are
  * (re, im) pairs.  DistMultiVector::axpby/scale/bdot,

@@ -18,7 +18,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
-HIP_SOURCES = ["matrix.hip", "apply.hip", "vector.hip", "kpm.hip", "spgemm.hip",
+HIP_SOURCES = ["matrix.hip", "apply.hip", "vector.hip", "kpm.hip", "spgemm.hip", "adjoint.hip",
                "lanes_f64.hip", "lanes_f32.hip", "lanes_c128.hip", "lanes_c64.hip", "lanes_f64m.hip",
                "lanes_c128m.hip"]
 LIB = os.path.join(HERE, "libvbcsr_hip.so")
