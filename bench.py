@@ -47,7 +47,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--quick", action="store_true", help="smaller lattice (smoke of the bench itself)")
-    p.add_argument("--only", default="", help="comma list: kpm,spmv,batched,cpu (default all)")
+    p.add_argument("--only", default="", help="comma list: kpm,spmv,batched,prop,mcweeny,cpu (default all)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a 1 GiB device-to-device copy 5 times (scale for the HBM byte counters)")
     return p.parse_args()
@@ -126,6 +126,119 @@ def cpu_baseline(G, K, M, seed):
     secs = time.perf_counter() - t0
     return {"value": Ms * K / secs, "unit": "moment-vecs/s", "cores": 1, "kind": "port",
             "sample": f"CPU restatement (oracle/, 1 thread), {Ms} moments x {K} vectors, FP64, {secs:.2f} s"}, None
+
+
+def bench_propagation(args, sync, log):
+    """C5: differentiable Chebyshev propagation on a 100x100-cell honeycomb
+    lattice, 4x4 complex blocks (N = 80,000), Gaussian wave packet of width
+    5 cells, 50 steps of dt = 0.5 with 64 Chebyshev terms, loss
+    |<psi_target|psi(T)>|^2, gradient with respect to every block value
+    (checkpoint every 10 steps), complex128 and complex64; central finite
+    differences on 20 random entries of the complex128 gradient."""
+    import torch
+
+    from vbcsr_amd import VBCSR
+    from vbcsr_amd import autograd as AG
+    from vbcsr_amd import workloads as W
+
+    nx = 30 if args.quick else 100
+    A = W.graphene(nx, nx, 4, seed=5, complex_blocks=True)
+    H = VBCSR.from_block_csr(A.bs, A.row_ptr, A.col_ind, A.vals)
+    psi0 = W.wavepacket(nx, nx, 4, 5.0, seed=6)
+    rng = np.random.default_rng(7)
+    tgt = rng.standard_normal(A.n) + 1j * rng.standard_normal(A.n)
+    tgt /= np.linalg.norm(tgt)
+    vals = torch.as_tensor(A.vals, device="cuda")
+    tg = torch.as_tensor(tgt, device="cuda")
+    dt, nsteps, nterms, half = 0.5, 50, 64, 1.0  # spectrum scaled to [-0.99, 0.99]
+
+    def loss_of(out):
+        ov = (tg.conj().to(out.dtype) * out).sum()
+        return ov.real ** 2 + ov.imag ** 2
+
+    res = {"workload": f"honeycomb {nx}x{nx} cells, 4x4 complex blocks (N={A.n}), {nsteps} steps dt={dt}, "
+                       f"{nterms} Chebyshev terms, checkpoint 10"}
+    grads = {}
+    for prec, cdt in (("fp64", torch.complex128), ("fp32", torch.complex64)):
+        p0 = torch.as_tensor(psi0, device="cuda").to(cdt)
+
+        def fwd():
+            with torch.no_grad():
+                return AG.propagate(H, vals, p0, dt, nsteps, nterms, half_width=half, precision=prec)
+
+        def fwd_bwd():
+            v = vals.clone().requires_grad_(True)
+            loss_of(AG.propagate(H, v, p0, dt, nsteps, nterms, half_width=half, checkpoint=10,
+                                 precision=prec)).backward()
+            grads[prec] = v.grad
+        tf = timed(fwd, 3, 1, sync)
+        tb = timed(fwd_bwd, 2, 1, sync)
+        norm = float(torch.linalg.norm(fwd().to(torch.complex128)).item())
+        res[prec] = {"forward_ms": tf * 1e3, "forward_backward_ms": tb * 1e3, "norm_psi_T": norm,
+                     "spmv_per_s_forward": nsteps * (nterms - 1) / tf}
+        log(f"propagation {prec}: forward {tf * 1e3:.1f} ms, forward+backward {tb * 1e3:.1f} ms, "
+            f"|psi(T)| = {norm:.12f}")
+    g64 = grads["fp64"].cpu().numpy()
+    res["grad_rel_err_fp32_vs_fp64"] = float(np.abs(grads["fp32"].cpu().numpy() - g64).max() / np.abs(g64).max())
+    p0 = torch.as_tensor(psi0, device="cuda")
+    eps, worst = 1e-6, 0.0
+    with torch.no_grad():
+        for idx in rng.choice(A.nnz, 20, replace=False):
+            for unit in (1.0, 1j):
+                vp, vm = vals.clone(), vals.clone()
+                vp[idx] += eps * unit
+                vm[idx] -= eps * unit
+                fd = (loss_of(AG.propagate(H, vp, p0, dt, nsteps, nterms, half_width=half)) -
+                      loss_of(AG.propagate(H, vm, p0, dt, nsteps, nterms, half_width=half))).item() / (2 * eps)
+                an = g64[idx].real if unit == 1.0 else g64[idx].imag
+                worst = max(worst, abs(fd - an) / max(abs(an), 1e-3 * np.abs(g64).max()))
+    res["grad_fd_max_rel_err_fp64"] = worst
+    log(f"propagation gradient: FD max rel err {worst:.2e}, fp32 vs fp64 {res['grad_rel_err_fp32_vs_fp64']:.2e}")
+    return res
+
+
+def bench_mcweeny(args, sync, log):
+    """C3: McWeeny purification P <- 3P^2 - 2P^3 with thresholded SpGEMM
+    (threshold 1e-5), 5000 block rows of sizes 4..8, ~30 neighbours per row,
+    P0 = 0.1 H + 0.5 I with H normalised by its Gershgorin bound (as written,
+    without it, P0's spectrum leaves [-0.5, 1.5] and the iteration diverges);
+    numeric phase in FP64, FP32 and BF16 (matrix cores); idempotency
+    ||P^2 - P||_F and trace in FP64.  An iteration that passes the time budget
+    ends the run for that precision (reported)."""
+    from vbcsr_amd import VBCSR
+    from vbcsr_amd import workloads as W
+
+    nb = 1000 if args.quick else 5000
+    iters, budget = 10, 60.0
+    P0 = W.mcweeny_start(nb=nb, seed=3, normalize=True)
+    res = {"workload": f"{nb} block rows, blocks 4-8, ~30 neighbours, Gershgorin-normalised H, threshold 1e-5, "
+                       f"up to {iters} iterations"}
+    for numeric in ("fp64", "fp32", "bf16"):
+        P = VBCSR.from_block_csr(P0.bs, P0.row_ptr, P0.col_ind, P0.vals)
+        sync()
+        t0 = time.perf_counter()
+        done = 0
+        per_iter = []
+        for it in range(iters):
+            ti = time.perf_counter()
+            P2 = P.spmm(P, 1e-5, numeric=numeric)
+            P3 = P2.spmm(P, 1e-5, numeric=numeric)
+            P = P2 * 3.0 - P3 * 2.0
+            sync()
+            per_iter.append(time.perf_counter() - ti)
+            done += 1
+            log(f"mcweeny {numeric} iteration {it}: {per_iter[-1] * 1e3:.1f} ms, {len(P.get_values())} values")
+            if time.perf_counter() - t0 > budget:
+                break
+        secs = time.perf_counter() - t0
+        P2 = P.spmm(P, 0.0)  # FP64 check: ||P^2 - P||_F over the union pattern, and tr P
+        idem = float(np.linalg.norm((P2 - P).get_values()))
+        res[numeric] = {"iterations": done, "seconds": secs, "ms_per_iteration": [t * 1e3 for t in per_iter],
+                        "trace": float(P.trace()), "values_final": int(len(P.get_values())),
+                        "idempotency_frobenius": idem}
+        log(f"mcweeny {numeric}: {done} iterations in {secs:.2f} s, trace {res[numeric]['trace']:.6f}, "
+            f"||P^2-P||_F {idem:.3e}")
+    return res
 
 
 def main():
@@ -287,6 +400,12 @@ def main():
                                     "speedup_vs_fp64_plain": t64 / tb[prec], "max_dos_error": errs[prec]}
             log(f"batched {prec}: {tb[prec] * 1e3:.2f} ms, speedup {t64 / tb[prec]:.2f}x vs fp64 plain "
                 f"{t64 * 1e3:.2f} ms, max DOS err {errs[prec]:.2e}")
+
+    if want("prop"):
+        out["propagation"] = bench_propagation(args, sync, log)
+
+    if want("mcweeny"):
+        out["mcweeny"] = bench_mcweeny(args, sync, log)
 
     cpu = None
     if want("cpu") and not args.no_cpu and not args.quick:
