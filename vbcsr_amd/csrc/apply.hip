@@ -145,10 +145,23 @@ static void launch_single(const vbh_matrix* m, const ApplyArgs<T>& a, hipStream_
 template <typename T, bool FUSED, typename V>
 void launch_rows(const vbh_matrix* m, const ApplyArgs<T, V>& a, hipStream_t s) {
     if (uses_lanes(a)) {
-        const LaneShape sh = lane_shape_of<T, V>(a.K);
-        ApplyArgs<T, V> b = a;
-        b.G = lane_groups(m->n_rows, sh.S);
-        launch_lanes<T, V>(b, m->n_rows, sh, m->uniform_bs == 4 ? 4 : 0, s);
+        LaneShape sh = lane_shape_of<T, V>(a.K);
+        const bool mf = uses_mfma(m, a);
+        int gl, gm;
+        split_groups(m, a, gl, gm, &sh.P);
+        if (gl > 0) {
+            ApplyArgs<T, V> b = a;
+            b.G = gl;
+            b.row_list = mf ? 1 : 0;
+            b.parts = sh.P;
+            launch_lanes<T, V>(b, mf ? m->n_small : m->n_rows, sh, m->uniform_bs == 4 ? 4 : 0, s);
+        }
+        if (mf) {
+            ApplyArgs<T, V> c = a;
+            c.G = gm;
+            c.partials = a.partials ? a.partials + 2 * int64_t(gl) : nullptr;
+            launch_mfma<T, V>(c, s);
+        }
         return;
     }
     if constexpr (is_mixed<T, V>()) {
@@ -182,6 +195,7 @@ template <typename T, typename V>
 ApplyArgs<T, V> make_args(const vbh_matrix* m, const V* vals, int K, const T* X, int64_t ldx, T* Y, int64_t ldy) {
     ApplyArgs<T, V> a{};
     a.p = m->pattern();
+    a.parts = 1;
     a.vals = vals;
     a.K = K;
     a.X = X;

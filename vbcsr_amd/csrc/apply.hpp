@@ -47,6 +47,8 @@ struct ApplyArgs {
     int64_t ldw;
     double* partials;  // [G][2*max(dot_cols,1)] per-work-group FP64 partial dots, or null
     int dot_cols;      // K: one partial per column; 1 (or 0, no dots): summed over the columns
+    int row_list;      // lane family: 1 = rows come from p.small_rows (the matrix-core class runs apart)
+    int parts;         // lane family: partitions of each row's slots (1, 2, 4, 8)
     const int* split;  // G+1 row chunk boundaries
     int G;
 };
@@ -219,7 +221,10 @@ __device__ inline void row_loop(const ApplyArgs<T>& g, int r0, int r1, int lane,
     }
 }
 
-constexpr int kSpmvTile(int rpp) { return 64 * rpp + 16; }
+// LDS tile of the SpMV per wave: a run's values (at most 64*rpp plus two
+// partial 16-byte words), then the run's 64 x entries (kSpmvX onwards).
+constexpr int kSpmvTile(int rpp) { return 64 * rpp + 16 + 64; }
+constexpr int kSpmvX(int rpp) { return 64 * rpp + 16; }
 
 // 16-byte vector of a storage type, for the coalesced LDS staging copies.
 template <typename T>
@@ -228,18 +233,65 @@ struct Vec16 {
     T v[16 / sizeof(T)];
 };
 
+// 16 raw bytes as a native vector type: staging words kept in registers
+// across loop iterations stay registers (arrays of structs would not).
+typedef int vbh_w4 __attribute__((ext_vector_type(4)));
+
+// Words of one staged run of the LDS SpMV per lane.
+template <typename T, int RPP>
+constexpr int spmv_words() {
+    return (kSpmvX(RPP) / Vec16<T>::n + 63) / 64;
+}
+
+// Issues the loads of one staged run into registers: its 16-byte value words
+// and one x entry per lane -- entry b of the run's block j/RPP for lane j, so
+// the run's x segments are read once, not once per row -- with clamped
+// (unconditional, in-bounds) indices: a guarded load compiles to a branch with
+// its own wait.
+template <typename T, int RPP>
+__device__ __forceinline__ void spmv_issue(const ApplyArgs<T>& g, int64_t vrow, int r, int nb, int ub, int mx, int mw,
+                                           int lane, vbh_w4 (&stage)[spmv_words<T, RPP>()], T& xone, int& lead,
+                                           int& nvec, int& pc_first, int& w, bool& act) {
+    constexpr int CH = 64 / RPP;
+    constexpr int EV = Vec16<T>::n;
+    const int a_l = lane & (RPP - 1);
+    const int p_l = lane / RPP;
+    const int ulast = (ub + CH < nb ? ub + CH : nb) - 1;
+    pc_first = lane_read(mw, ub) >> 8;
+    const int wl = lane_read(mw, ulast);
+    const int pc_end = (wl >> 8) + (wl & 255);
+    const int64_t s0 = vrow + int64_t(r) * pc_first;  // first value of the run
+    const int64_t sa = s0 & ~int64_t(EV - 1);         // 16-byte aligned start
+    lead = int(s0 - sa);
+    nvec = (lead + r * (pc_end - pc_first) + EV - 1) / EV;
+    const vbh_w4* src = reinterpret_cast<const vbh_w4*>(g.vals + sa);
+    const int u = ub + p_l;
+    const int xo = __shfl(mx, u, 64);  // lanes past the row's slots hold x offset 0, width 0
+    w = __shfl(mw, u, 64);
+    act = u <= ulast && a_l < r;
+#pragma unroll
+    for (int q = 0; q < spmv_words<T, RPP>(); ++q) {
+        const int e = q * 64 + lane;
+        stage[q] = src[e < nvec ? e : nvec - 1];
+    }
+    // lane = p*RPP + b loads x[xo_p + b] (clamped into the block's width)
+    const int c = w & 255;
+    const int cm = c > 0 ? c - 1 : 0;
+    xone = g.X[int64_t(xo + (a_l < cm ? a_l : cm)) * g.ldx];
+}
+
 // SpMV row loop for blocks up to RPP x RPP (RPP in {1, 2, 4, 8}); lds is this
 // wave's 16-byte aligned tile of kSpmvTile(RPP) elements: a run of 64/RPP
 // blocks holds at most 64*RPP values, plus up to two partial 16-byte words.
+// Software-pipelined within each 64-slot chunk of a row: run n+1's loads are
+// in flight while run n is computed from the tile.
 template <typename T, int RPP, bool FUSED>
 __device__ inline void row_loop_spmv(const ApplyArgs<T>& g, int r0, int r1, int lane, int wave, double& d1,
                                      double& d2, T* lds) {
     using A = typename Wide<T>::type;
     constexpr int CH = 64 / RPP;  // blocks per staged run = partitions
-    constexpr int EV = Vec16<T>::n;
     const DevPattern& p = g.p;
     const int a_l = lane & (RPP - 1);
-    const int p_l = lane / RPP;
     for (int i = r0 + wave; i < r1; i += kWaves) {
         const int r = p.rbs[i];
         const int kb = p.row_ptr[i], ke = p.row_ptr[i + 1];
@@ -248,61 +300,45 @@ __device__ inline void row_loop_spmv(const ApplyArgs<T>& g, int r0, int r1, int 
         for (int t0 = kb; t0 < ke; t0 += 64) {
             const int nb = ke - t0 < 64 ? ke - t0 : 64;
             int mx = 0, mw = 0;
-            if (lane < nb) {
-                const int2 mm = p.meta[t0 + lane];
-                mx = mm.x;
-                mw = mm.y;
+            {
+                const int2 mm = p.meta[t0 + (lane < nb ? lane : nb - 1)];
+                mx = lane < nb ? mm.x : 0;
+                mw = lane < nb ? mm.y : 0;
             }
+            constexpr int NV = spmv_words<T, RPP>();
+            vbh_w4 stage[NV];
+            T xone;
+            int lead = 0, nvec = 1, pc_first = 0, w = 0;
+            bool act = false;
+            spmv_issue<T, RPP>(g, vrow, r, nb, 0, mx, mw, lane, stage, xone, lead, nvec, pc_first, w, act);
             for (int ub = 0; ub < nb; ub += CH) {
-                const int ulast = (ub + CH < nb ? ub + CH : nb) - 1;
-                const int pc_first = lane_read(mw, ub) >> 8;
-                const int wl = lane_read(mw, ulast);
-                const int pc_end = (wl >> 8) + (wl & 255);
-                const int64_t s0 = vrow + int64_t(r) * pc_first;      // first value of the run
-                const int64_t sa = s0 & ~int64_t(EV - 1);             // 16-byte aligned start
-                const int lead = int(s0 - sa);
-                const int nvec = (lead + r * (pc_end - pc_first) + EV - 1) / EV;
-                const Vec16<T>* src = reinterpret_cast<const Vec16<T>*>(g.vals + sa);
-                Vec16<T>* dst = reinterpret_cast<Vec16<T>*>(lds);
-                const int u = ub + p_l;
-                const int xo = __shfl(mx, u, 64);
-                const int w = __shfl(mw, u, 64);
-                const bool act = u <= ulast && a_l < r;
-                const int c = w & 255;
-                // every load of the run is issued before any is consumed: the
-                // run's values (NV 16-byte words per lane) and this lane's x
-                // entries (c <= RPP), so the wave pays one memory latency per run
-                // Indices are clamped into range instead of guarded: a guarded
-                // load compiles to a branch with its own wait, which would
-                // serialise the run again.
-                constexpr int NV = (kSpmvTile(RPP) / EV + 63) / 64;
-                Vec16<T> stage[NV];
+                // this run's words to the tile (lanes past the run rewrite word
+                // nvec-1 with the value it already holds), its x to xc
+                {
+                    vbh_w4* dstw = reinterpret_cast<vbh_w4*>(lds);
 #pragma unroll
-                for (int q = 0; q < NV; ++q) {
-                    const int e = q * 64 + lane;
-                    stage[q] = src[e < nvec ? e : nvec - 1];
+                    for (int q = 0; q < NV; ++q) {
+                        const int e = q * 64 + lane;
+                        dstw[e < nvec ? e : nvec - 1] = stage[q];
+                    }
                 }
-                T xv[RPP];
-                const T* xb = g.X + int64_t(xo) * g.ldx;  // xo is a valid row for every lane
-                const int cm = c > 0 ? c - 1 : 0;
-#pragma unroll
-                for (int b = 0; b < RPP; ++b) xv[b] = xb[int64_t(b < cm ? b : cm) * g.ldx];
-                // unconditional stores too (a guarded store lets the compiler sink
-                // its load into the branch): lanes past the run rewrite word
-                // nvec-1 with the value it already holds
-#pragma unroll
-                for (int q = 0; q < NV; ++q) {
-                    const int e = q * 64 + lane;
-                    dst[e < nvec ? e : nvec - 1] = stage[q];
-                }
+                lds[kSpmvX(RPP) + lane] = xone;
+                const int lead_c = lead, pc_first_c = pc_first, w_c = w;
+                const bool act_c = act;
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                if (act) {
-                    const T* ab = lds + lead + r * ((w >> 8) - pc_first) + a_l * c;
+                if (ub + CH < nb)  // overlaps the FMAs below
+                    spmv_issue<T, RPP>(g, vrow, r, nb, ub + CH, mx, mw, lane, stage, xone, lead, nvec, pc_first, w,
+                                       act);
+                if (act_c) {
+                    const int c = w_c & 255;
+                    const int cm = c > 0 ? c - 1 : 0;
+                    const T* ab = lds + lead_c + r * ((w_c >> 8) - pc_first_c) + a_l * c;
+                    const T* xs = lds + kSpmvX(RPP) + (lane & ~(RPP - 1));  // this block's x entries
 #pragma unroll
                     for (int b = 0; b < RPP; ++b) {
-                        const A t = Sc<A>::fma(widen(ab[b < cm ? b : cm]), widen(xv[b]), acc);
+                        const A t = Sc<A>::fma(widen(ab[b < cm ? b : cm]), widen(xs[b]), acc);
                         acc = b < c ? t : acc;
                     }
                 }
@@ -353,6 +389,7 @@ RowConfig row_config(int K, int max_block, int uniform_bs);
 // Lane-per-row family (lanes.hpp): S lanes per scalar row, KT columns each.
 struct LaneShape {
     int S, KT;  // S == 0: family not applicable
+    int P = 1;  // partitions of each row's slots (lane_parts)
 };
 // Instantiated shapes: real K <= 16 one lane per row (KT in 4, 8, 10, 12, 16),
 // up to 64 with 2 or 4 lanes of 16 columns; complex KT = 8 with 1..8 lanes.
@@ -370,10 +407,26 @@ inline LaneShape lane_shape(bool complex, int K, bool mixed = false) {
     return {K <= 32 ? 2 : 4, 16};
 }
 // Scalar rows handled by one work-group of the lane family.
-inline int lane_rows_per_group(int S) { return kWaves * (64 / S); }
+inline int lane_rows_per_group(int S, int P = 1) { return kWaves * (64 / (S * P)); }
+// Partitions of each row's slots: small launches split rows over more lanes,
+// so every lane's dependent chain of block loads gets shorter and the chip
+// fills.  Target: 4 waves on each of 1024 SIMDs for 4-byte vectors, 8 for
+// wider ones (measured on the 200x200 graphene KPM step: FP32 is fastest at
+// P = 1, FP64 at P = 2).  Quads (4x4 blocks) need 4 rows per lane group.
+inline int lane_parts(int64_t rows, int S, int bs, int elem_bytes) {
+    const int64_t target = int64_t(256) * 4 * (elem_bytes <= 4 ? 4 : 8) * 64;
+    const int pmax = bs == 4 ? 16 / S : (64 / S < 8 ? 64 / S : 8);
+    int P = 1;
+#ifdef VBH_FORCE_PARTS
+    while (P * 2 <= pmax && P < VBH_FORCE_PARTS) P *= 2;
+    return P;
+#endif
+    while (P * 2 <= pmax && rows * S * P < target) P *= 2;
+    return P;
+}
 // Work-groups of the lane family over n rows (a multiple of 8 for the XCD map).
-inline int lane_groups(int64_t n_rows, int S) {
-    const int64_t rpg = lane_rows_per_group(S);
+inline int lane_groups(int64_t n_rows, int S, int P = 1) {
+    const int64_t rpg = lane_rows_per_group(S, P);
     const int64_t g = (n_rows + rpg - 1) / rpg;
     return int(((g < 8 ? 8 : g) + 7) & ~int64_t(7));
 }
@@ -407,16 +460,53 @@ template <typename T, typename V>
 inline bool uses_lanes(const ApplyArgs<T, V>& a) {
     return lane_shape_of<T, V>(a.K).S > 0 && a.dot_cols <= 1;
 }
+// Matrix-core class: block rows with kMfmaMinRows..16 rows of a real matrix
+// run as 16x16 FP64 MFMA tiles, the other rows on the lane family.
+constexpr int kMfmaRowsPerGroup = kWaves;  // block rows per work-group (one per wave)
+inline int mfma_groups(int n_big) {
+    const int g = (n_big + kMfmaRowsPerGroup - 1) / kMfmaRowsPerGroup;
+    return ((g < 8 ? 8 : g) + 7) & ~7;
+}
+// The LDS tile holds blocks up to 16 x 16 and reads X rows as aligned 16-byte
+// words (K a multiple of 16/sizeof(T), X and ldx aligned to match).
+template <typename T>
+inline bool mfma_x_ok(const void* X, int64_t ldx, int K) {
+    const int ev = int(16 / sizeof(T));
+    return K % ev == 0 && ldx % ev == 0 && reinterpret_cast<uintptr_t>(X) % 16 == 0;
+}
+template <typename T, typename V>
+inline bool uses_mfma(const vbh_matrix* m, const ApplyArgs<T, V>& a) {
+    return !Sc<T>::is_complex && m->n_big > 0 && m->max_block <= 16 && uses_lanes(a) &&
+           mfma_x_ok<T>(a.X, a.ldx, a.K);
+}
+// Work-groups of the lane part (over the small rows when the matrix-core
+// class runs apart) and of the matrix-core part.
+template <typename T, typename V>
+inline void split_groups(const vbh_matrix* m, const ApplyArgs<T, V>& a, int& g_lanes, int& g_mfma, int* parts = nullptr) {
+    const bool mf = uses_mfma(m, a);
+    const int64_t rows = mf ? m->n_small : m->n_rows;
+    const int S = lane_shape_of<T, V>(a.K).S;
+    const int P = lane_parts(rows, S, m->uniform_bs == 4 ? 4 : 0, int(sizeof(T)));
+    if (parts) *parts = P;
+    g_lanes = rows > 0 ? lane_groups(rows, S, P) : 0;
+    g_mfma = mf ? mfma_groups(m->n_big) : 0;
+}
 // Work-groups (= partial slots) a launch_rows call with these arguments uses.
 template <typename T, typename V>
 inline int apply_groups(const vbh_matrix* m, const ApplyArgs<T, V>& a) {
-    return uses_lanes(a) ? lane_groups(m->n_rows, lane_shape_of<T, V>(a.K).S) : m->G;
+    if (!uses_lanes(a)) return m->G;
+    int gl, gm;
+    split_groups(m, a, gl, gm);
+    return gl + gm;
 }
 
 template <typename T, bool FUSED, typename V = T>
 void launch_rows(const vbh_matrix* m, const ApplyArgs<T, V>& a, hipStream_t s);
 template <typename T, typename V>
 void launch_lanes(const ApplyArgs<T, V>& a, int64_t n_rows, LaneShape sh, int bs, hipStream_t s);
+// Matrix-core class (a.G groups over p.big_brows); real types only.
+template <typename T, typename V>
+void launch_mfma(const ApplyArgs<T, V>& a, hipStream_t s);
 
 // One matrix of a batched KPM launch.
 template <typename T, typename V = T>
@@ -427,13 +517,19 @@ struct BatchItem {
     const T* v0;   // mode 1: the start vectors (W of the dot)
     const int* split;
     int64_t n_rows;
-    int G;         // this matrix's work-groups
-    int g_base;    // first global work-group
+    int64_t lane_rows;  // rows of the lane launch: n_rows, or the small rows (row_list)
+    int row_list;       // 1: the lane launch takes p.small_rows, the matrix-core launch p.big_brows
+    int G;              // this matrix's lane (or row-loop) work-groups
+    int g_base;         // its first work-group in the lane launch
+    int g_base2;        // its first work-group in the matrix-core launch
 };
 template <typename T, typename V>
 void launch_batched_lanes(const BatchItem<T, V>* items, int nitems, int Gtot, int K, int ld, LaneShape sh, int step,
                           double alpha, double beta, double gamma0, double gamma, int mode, double* partials,
                           hipStream_t s);
+template <typename T, typename V>
+void launch_batched_mfma(const BatchItem<T, V>* items, int nitems, int Gtot, int K, int ld, int step, double alpha,
+                         double beta, double gamma0, double gamma, int mode, double* partials, hipStream_t s);
 template <typename T, typename V = T>
 ApplyArgs<T, V> make_args(const vbh_matrix* m, const V* vals, int K, const T* X, int64_t ldx, T* Y, int64_t ldy);
 template <typename T>

@@ -182,9 +182,17 @@ struct DevPattern {
     // lane-parallel load yields every block's x offset, width and values.
     const int2* meta;
     const int* row_of;  // n_rows: block row of every scalar row
+    // row classes of the real-type SpMM: block rows with kMfmaMinRows <= r <= 16
+    // (matrix-core tiles, big_brows) and the scalar rows of all other block
+    // rows (lane family, small_rows); n_big == 0 when there are none
+    const int* small_rows;
+    const int* big_brows;
+    int64_t n_small;
+    int n_big;
 };
 
 constexpr int kWaves = 4;  // waves per work-group in the row kernels
+constexpr int kMfmaMinRows = 8;  // block rows of 8..16 rows run on 16x16x4 FP64 matrix-core tiles
 constexpr int kThreads = 64 * kWaves;
 constexpr int kVecPad = 8;  // padding elements after every value array
 
@@ -204,6 +212,9 @@ struct vbh_matrix {
     int64_t* voff = nullptr;
     int2* meta = nullptr;
     int* row_of = nullptr;
+    int *small_rows = nullptr, *big_brows = nullptr;
+    int64_t n_small = 0;
+    int n_big = 0;
     // values
     void* vals = nullptr;
     void* vals32 = nullptr;  // FP32 copy, built on demand
@@ -217,7 +228,8 @@ struct vbh_matrix {
     bool t_valid = false;
 
     vbh::DevPattern pattern() const {
-        return vbh::DevPattern{nbr, nbc, uniform_bs, rbs, cbs, row_ptr, col_ind, roff, coff, voff, meta, row_of};
+        return vbh::DevPattern{nbr,  nbc,    uniform_bs, rbs,        cbs,       row_ptr, col_ind, roff,
+                               coff, voff,   meta,       row_of,     small_rows, big_brows, n_small, n_big};
     }
     size_t elem_size() const { return dtype == VBH_C128 ? 16 : 8; }
 };

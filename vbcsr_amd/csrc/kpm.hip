@@ -72,12 +72,17 @@ __global__ __launch_bounds__(256) void k_redot_partials(int64_t rows, int K, con
 
 // Sum partials per (step, segment): out[(s*nseg + h)*E + e] =
 // sum_{g in seg h} partials[s][g][e]; grid (E, nseg, nsteps).
+// Segment h also owns [off2 + seg2[h], off2 + seg2[h+1]) when seg2 is given
+// (the matrix-core groups of a batch, stored after the lane groups).
 __global__ void k_reduce_segments(const double* __restrict__ partials, int Gtot, int E, const int* __restrict__ seg,
-                                  int nseg, double* __restrict__ out) {
+                                  int nseg, double* __restrict__ out, const int* __restrict__ seg2, int off2) {
     const int e = blockIdx.x, h = blockIdx.y, s = blockIdx.z;
     const double* base = partials + int64_t(s) * Gtot * E;
     double acc = 0.0;
     for (int g = seg[h] + threadIdx.x; g < seg[h + 1]; g += blockDim.x) acc += base[int64_t(g) * E + e];
+    if (seg2)
+        for (int g = off2 + seg2[h] + threadIdx.x; g < off2 + seg2[h + 1]; g += blockDim.x)
+            acc += base[int64_t(g) * E + e];
     __shared__ double red[256];
     red[threadIdx.x] = acc;
     __syncthreads();
@@ -282,7 +287,8 @@ static void kpm_single(vbh_matrix* m, int K, const S* V0, int ldv, int M, int mo
     const int hseg[2] = {0, G};
     int* dseg = (int*)scratch(sizeof(int) * 2, 7);
     VBH_CHECK_HIP(hipMemcpyAsync(dseg, hseg, sizeof(int) * 2, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_reduce_segments, dim3(E, 1, unsigned(nsteps)), dim3(256), 0, s, partials, G, E, dseg, 1, red);
+    hipLaunchKernelGGL(k_reduce_segments, dim3(E, 1, unsigned(nsteps)), dim3(256), 0, s, partials, G, E, dseg, 1, red,
+                       (const int*)nullptr, 0);
     VBH_CHECK_HIP(hipGetLastError());
     std::vector<double> d(nsteps * E);
     VBH_CHECK_HIP(hipMemcpyAsync(d.data(), red, sizeof(double) * nsteps * E, hipMemcpyDeviceToHost, s));
@@ -314,19 +320,34 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
     VBH_REQUIRE(K >= 1 && K <= 64, "batched KPM supports 1 <= K <= 64");
     const int64_t nsteps = step_count<T>(M, mode);
     std::vector<BatchItem<T, V>> items(count);
-    std::vector<int> seg(count + 1, 0);
+    std::vector<int> seg(count + 1, 0), seg2(count + 1, 0);
     std::vector<size_t> off(count + 1, 0);
-    // K >= 2: lane family, G_h = ceil(rows / rows-per-group); K == 1: row loop
-    // over each matrix's work-balanced split.  Dots are summed over the
-    // columns (E = 2) in both.
-    const LaneShape sh = lane_shape_of<T, V>(K);
+    // K >= 2: lane family, G_h = ceil(rows / rows-per-group) -- over the small
+    // rows when the matrix has a matrix-core class (real types), whose block
+    // rows run as a second launch with G2_h = ceil(big rows / 16); K == 1: row
+    // loop over each matrix's work-balanced split.  Dots are summed over the
+    // columns (E = 2) everywhere.
+    LaneShape sh = lane_shape_of<T, V>(K);
+    bool mfma_ok = !Sc<T>::is_complex && sh.S > 0 && mfma_x_ok<T>(nullptr, K, K);  // buffers: ld = K, 256-B aligned
+    for (int h = 0; h < count; ++h) mfma_ok = mfma_ok && ms[h]->max_block <= 16;
+    if (sh.S) {  // slot partitions from the batch's total lane rows
+        int64_t lrows = 0;
+        for (int h = 0; h < count; ++h)
+            lrows += (mfma_ok && ms[h]->n_big > 0) ? ms[h]->n_small : ms[h]->n_rows;
+        sh.P = lane_parts(lrows, sh.S, 0, int(sizeof(T)));
+    }
     for (int h = 0; h < count; ++h) {
         VBH_REQUIRE(ms[h]->nbc == ms[h]->nbr, "batched KPM needs square local patterns without ghosts");
         off[h + 1] = off[h] + vec_bytes_of<T>(ms[h]->n_rows, K) * (mode == 1 ? 4 : 3);
-        const int64_t rpg = sh.S ? lane_rows_per_group(sh.S) : 1;
-        seg[h + 1] = seg[h] + (sh.S ? int((ms[h]->n_rows + rpg - 1) / rpg) : ms[h]->G);
+        const int64_t rpg = sh.S ? lane_rows_per_group(sh.S, sh.P) : 1;
+        const bool split = mfma_ok && ms[h]->n_big > 0;
+        const int64_t lrows = split ? ms[h]->n_small : ms[h]->n_rows;
+        seg[h + 1] = seg[h] + (sh.S ? int((lrows + rpg - 1) / rpg) : ms[h]->G);
+        seg2[h + 1] = seg2[h] + (split ? (ms[h]->n_big + kMfmaRowsPerGroup - 1) / kMfmaRowsPerGroup : 0);
     }
-    const int Gtot = (seg[count] + 7) & ~7;  // trailing groups find no rows
+    const int Gtot1 = (seg[count] + 7) & ~7;  // trailing groups find no rows
+    const int Gtot2 = (seg2[count] + 7) & ~7;
+    const int Gtot = Gtot1 + Gtot2;
     const int E = 2;
     char* vecs = (char*)scratch(std::max<size_t>(off[count], 64), 4);
     double* partials = (double*)scratch(sizeof(double) * std::max<int64_t>(nsteps, 1) * Gtot * E, 5);
@@ -343,8 +364,11 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
         it.v0 = mode == 1 ? (const T*)(vecs + off[h] + 3 * vb) : nullptr;
         it.split = m->split;
         it.n_rows = N;
+        it.row_list = (mfma_ok && m->n_big > 0) ? 1 : 0;
+        it.lane_rows = it.row_list ? m->n_small : N;
         it.G = seg[h + 1] - seg[h];
         it.g_base = seg[h];
+        it.g_base2 = seg2[h];
         if (N * K > 0) {
             hipLaunchKernelGGL((k_copy_convert<S, T>), dim3(unsigned((N * K + 255) / 256)), dim3(256), 0, s, N, K,
                                (const S*)V0s[h], int64_t(ldv[h]), it.buf[0], int64_t(K));
@@ -361,10 +385,12 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
     }
     if (nsteps > 0 && Gtot > 0) {
         using Item = BatchItem<T, V>;
-        Item* ditems = (Item*)scratch(sizeof(Item) * count + sizeof(int) * (count + 1), 6);
+        Item* ditems = (Item*)scratch(sizeof(Item) * count + 2 * sizeof(int) * (count + 1), 6);
         int* dseg = (int*)(ditems + count);
+        int* dseg2 = dseg + (count + 1);
         VBH_CHECK_HIP(hipMemcpyAsync(ditems, items.data(), sizeof(Item) * count, hipMemcpyHostToDevice, s));
         VBH_CHECK_HIP(hipMemcpyAsync(dseg, seg.data(), sizeof(int) * (count + 1), hipMemcpyHostToDevice, s));
+        VBH_CHECK_HIP(hipMemcpyAsync(dseg2, seg2.data(), sizeof(int) * (count + 1), hipMemcpyHostToDevice, s));
         VBH_CHECK_HIP(hipStreamSynchronize(s));
         const double a1 = 1.0 / half, b1 = -center / half;
         const double a2 = 2.0 / half, b2 = -2.0 * center / half;
@@ -377,7 +403,10 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
                 const double al = step == 0 ? a1 : a2, be = step == 0 ? b1 : b2;
                 double* pt = partials + step * Gtot * E;
                 if (sh.S) {
-                    launch_batched_lanes<T, V>(it, count, Gtot, K, K, sh, int(step), al, be, 0.0, -1.0, mode, pt, st);
+                    launch_batched_lanes<T, V>(it, count, Gtot1, K, K, sh, int(step), al, be, 0.0, -1.0, mode, pt, st);
+                    if (Gtot2 > 0)
+                        launch_batched_mfma<T, V>(it, count, Gtot2, K, K, int(step), al, be, 0.0, -1.0, mode,
+                                                  pt + int64_t(Gtot1) * E, st);
                     continue;
                 }
                 if constexpr (!is_mixed<T, V>()) {  // K == 1, single type: row loop
@@ -405,7 +434,7 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
             } cache;
             std::vector<uintptr_t> key = {uintptr_t(ditems), uintptr_t(vecs), uintptr_t(partials), uintptr_t(count),
                                           uintptr_t(K), uintptr_t(nsteps), uintptr_t(mode), uintptr_t(sizeof(T)),
-                                          uintptr_t(Gtot)};
+                                          uintptr_t(Gtot), uintptr_t(Gtot1), uintptr_t(sh.P)};
             double ab[4] = {a1, b1, a2, b2};
             for (double v : ab) {
                 uint64_t bits;
@@ -441,7 +470,7 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
         }
         double* red = (double*)scratch(sizeof(double) * nsteps * count * E, 7);
         hipLaunchKernelGGL(k_reduce_segments, dim3(E, count, unsigned(nsteps)), dim3(256), 0, s, partials, Gtot, E,
-                           dseg, count, red);
+                           dseg, count, red, Gtot2 > 0 ? (const int*)dseg2 : nullptr, Gtot1);
         VBH_CHECK_HIP(hipGetLastError());
         std::vector<double> d(nsteps * count * E);
         VBH_CHECK_HIP(hipMemcpyAsync(d.data(), red, sizeof(double) * d.size(), hipMemcpyDeviceToHost, s));

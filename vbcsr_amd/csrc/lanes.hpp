@@ -123,8 +123,8 @@ __device__ __forceinline__ V quad_bcast(V v) {
 // with the slice width to keep 5+ waves per SIMD resident (measured: U = 2
 // beats 1 and 4 for 10 FP32 columns).
 template <typename T, typename V, int KT, int VN>
-__device__ __forceinline__ void quad_slots(const ApplyArgs<T, V>& g, int kb, int ke, int a, int v0, int nvalid,
-                                           typename Wide<T>::type (&acc)[KT]) {
+__device__ __forceinline__ void quad_slots(const ApplyArgs<T, V>& g, int kb, int cnt, int stride, int a, int v0,
+                                           int nvalid, typename Wide<T>::type (&acc)[KT]) {
     using A = typename Wide<T>::type;
     constexpr int CB = 4;                                   // slots per lane per pass
     constexpr int per = int(KT * sizeof(T) / 4);            // VGPRs of one X slice
@@ -134,14 +134,16 @@ __device__ __forceinline__ void quad_slots(const ApplyArgs<T, V>& g, int kb, int
     constexpr int U = per <= 8 ? 4 : per <= 24 ? 2 : 1;     // slots per batch
 #endif
     const DevPattern& p = g.p;
-    for (int k0 = kb; k0 < ke; k0 += 4 * CB) {
-        // ns >= 1; slot indices past the row are clamped to its last slot, so
-        // every load below is unconditional and in bounds, and the extra
-        // slots' products are dropped by a select
-        const int ns = ke - k0 < 4 * CB ? ke - k0 : 4 * CB;
+    // logical slot j of this lane's partition is physical slot kb + j*stride
+    for (int j0 = 0; j0 < cnt; j0 += 4 * CB) {
+        // ns >= 1; slot indices past the partition are clamped to its last
+        // slot, so every load below is unconditional and in bounds, and the
+        // extra slots' products are dropped by a select
+        const int ns = cnt - j0 < 4 * CB ? cnt - j0 : 4 * CB;
+        const int k0 = kb + j0 * stride;
         int colv[CB];
 #pragma unroll
-        for (int t = 0; t < CB; ++t) colv[t] = p.col_ind[k0 + (a + 4 * t < ns ? a + 4 * t : ns - 1)];
+        for (int t = 0; t < CB; ++t) colv[t] = p.col_ind[k0 + (a + 4 * t < ns ? a + 4 * t : ns - 1) * stride];
 #pragma unroll
         for (int s0 = 0; s0 < 4 * CB; s0 += U) {
             if (s0 >= ns) break;
@@ -158,8 +160,8 @@ __device__ __forceinline__ void quad_slots(const ApplyArgs<T, V>& g, int kb, int
                     default: col = quad_bcast<3>(colv[s >> 2]); break;
                 }
                 const int sc = s < ns ? s : ns - 1;
-                const V* ar = static_cast<const V*>(
-                    __builtin_assume_aligned(g.vals + int64_t(k0 + sc) * 16 + a * 4, (4 * sizeof(V)) % 16 ? sizeof(V) : 16));
+                const V* ar = static_cast<const V*>(__builtin_assume_aligned(
+                    g.vals + int64_t(k0 + sc * stride) * 16 + a * 4, (4 * sizeof(V)) % 16 ? sizeof(V) : 16));
 #pragma unroll
                 for (int b = 0; b < 4; ++b) av[u][b] = ar[b];
                 load_slice<T, KT, VN>(g.X + (int64_t(col) * 4 + a) * g.ldx + v0, nvalid, xr[u]);
@@ -190,13 +192,21 @@ template <typename T, typename V, int BS, int S, int KT, int VN, bool DOT>
 __device__ __forceinline__ void lane_rows(const ApplyArgs<T, V>& g, int64_t sb, int64_t se, int lane, int wave,
                                           double& d1, double& d2) {
     using A = typename Wide<T>::type;
-    constexpr int R = 64 / S;
     const DevPattern& p = g.p;
-    const int v0 = (lane / R) * KT;
+    // lane = (part * S + slice) * R + row: P partitions of each row's slots
+    const int P = g.parts;
+    const int R = 64 / (S * P);
+    const int grp = lane / R;
+    const int part = grp / S;
+    const int v0 = (grp - part * S) * KT;
     const int nvalid = g.K - v0;
     for (int64_t base = sb + int64_t(wave) * R; base < se; base += int64_t(kWaves) * R) {
-        const int64_t srow = base + lane % R;
-        const bool live = srow < se && nvalid > 0;
+        const int64_t idx = base + lane % R;
+        const bool live = idx < se && nvalid > 0;
+        int64_t srow = idx;
+        if constexpr (BS == 0) {
+            if (g.row_list && live) srow = p.small_rows[idx];
+        }
         int kb = 0, ke = 0, r = BS, a = 0;
         int64_t vrow = 0, xown = 0;
         if (live) {
@@ -221,9 +231,10 @@ __device__ __forceinline__ void lane_rows(const ApplyArgs<T, V>& g, int64_t sb, 
 #pragma unroll
         for (int j = 0; j < KT; ++j) acc[j] = Sc<A>::zero();
         if constexpr (BS == 4) {
-            quad_slots<T, V, KT, VN>(g, kb, ke, a, v0, nvalid, acc);
+            const int kp = kb + part;
+            quad_slots<T, V, KT, VN>(g, kp, ke > kp ? (ke - kp + P - 1) / P : 0, P, a, v0, nvalid, acc);
         } else if constexpr (BS > 0) {
-            for (int k = kb; k < ke; ++k) {
+            for (int k = kb + part; k < ke; k += P) {
                 const int col = p.col_ind[k];
                 const V* ar = static_cast<const V*>(
                     __builtin_assume_aligned(g.vals + vrow + int64_t(k - kb) * (BS * BS), (BS * sizeof(V)) % 16 ? sizeof(V) : 16));
@@ -250,10 +261,11 @@ __device__ __forceinline__ void lane_rows(const ApplyArgs<T, V>& g, int64_t sb, 
 #else
             constexpr int BU = per <= 8 ? 4 : 2;
 #endif
-            int2 mn = p.meta[kb < ke ? kb : 0];
-            for (int k = kb; k < ke; ++k) {
+            const int kp = kb + part;
+            int2 mn = p.meta[kp < ke ? kp : 0];
+            for (int k = kp; k < ke; k += P) {
                 const int2 mm = mn;
-                mn = p.meta[k + 1 < ke ? k + 1 : k];
+                mn = p.meta[k + P < ke ? k + P : k];
                 const int c = mm.y & 255;
                 const V* ar = g.vals + vrow + int64_t(r) * (mm.y >> 8) + a * c;
                 const T* xb = g.X + int64_t(mm.x) * g.ldx + v0;
@@ -279,14 +291,19 @@ __device__ __forceinline__ void lane_rows(const ApplyArgs<T, V>& g, int64_t sb, 
                 }
             }
         }
-        if (live) emit_slice<T, V, KT, VN, DOT>(g, srow, xown, v0, nvalid, acc, d1, d2);
+        // merge the partitions (lanes 64/P apart) into part 0
+        for (int m = 64 / P; m < 64; m <<= 1) {
+#pragma unroll
+            for (int j = 0; j < KT; ++j) acc[j] = acc[j] + shfl_xor(acc[j], m);
+        }
+        if (live && part == 0) emit_slice<T, V, KT, VN, DOT>(g, srow, xown, v0, nvalid, acc, d1, d2);
     }
 }
 
 template <typename T, typename V, int BS, int S, int KT, int VN, bool DOT>
 __global__ __launch_bounds__(kThreads)
 void k_apply_lanes(ApplyArgs<T, V> g, int64_t n_rows) {
-    constexpr int RPG = kWaves * (64 / S);
+    const int RPG = kWaves * (64 / (S * g.parts));
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
     const int chunk = xcd_chunk(g.G);
@@ -297,30 +314,16 @@ void k_apply_lanes(ApplyArgs<T, V> g, int64_t n_rows) {
     if constexpr (DOT) write_partials(1, lane, wave, d1, d2, g.partials + int64_t(chunk) * 2);
 }
 
-// One KPM recursion step over every matrix of a batch (see kpm.hip): work-group
-// gg finds its matrix by binary search over g_base; groups past the last
-// matrix's rows write zero partials.
-template <typename T, typename V, int S, int KT, int VN>
-__global__ __launch_bounds__(kThreads)
-void k_batched_lanes(const BatchItem<T, V>* __restrict__ items, int nitems,
-                                                            int Gtot, int K, int ld, int step, double alpha,
-                                                            double beta, double gamma0, double gamma, int mode,
-                                                            double* __restrict__ partials) {
-    constexpr int RPG = kWaves * (64 / S);
-    const int lane = threadIdx.x & 63;
-    const int wave = threadIdx.x >> 6;
-    const int gg = xcd_chunk(Gtot);
-    int lo = 0, hi = nitems - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (items[mid].g_base <= gg) lo = mid;
-        else hi = mid - 1;
-    }
-    lo = uniform(lo);  // the same for the whole work-group: scalar loads below
-    // The item table is read through the constant address space: it is
-    // read-only during the launch, and the compiler then takes the pointers
-    // loaded from it to be global (global_load rather than flat_load, whose
-    // completion also holds the LDS/scalar wait counter).
+// Apply arguments of batch item lo for one recursion step (buffers rotate
+// t_{n-1}, t_n, t_{n+1} through buf[3]).  The item table is read through the
+// constant address space: it is read-only during the launch, and the
+// compiler then takes the pointers loaded from it to be global (global_load
+// rather than flat_load, whose completion also holds the LDS/scalar wait
+// counter).
+template <typename T, typename V>
+__device__ __forceinline__ ApplyArgs<T, V> batch_args(const BatchItem<T, V>* items, int lo, int K, int ld, int step,
+                                                      double alpha, double beta, double gamma0, double gamma,
+                                                      int mode) {
     using CItem = const __attribute__((address_space(4))) BatchItem<T, V>;
     CItem& it = ((CItem*)items)[lo];
     ApplyArgs<T, V> a{};
@@ -336,6 +339,10 @@ void k_batched_lanes(const BatchItem<T, V>* __restrict__ items, int nitems,
     a.p.voff = it.p.voff;
     a.p.meta = it.p.meta;
     a.p.row_of = it.p.row_of;
+    a.p.small_rows = it.p.small_rows;
+    a.p.big_brows = it.p.big_brows;
+    a.p.n_small = it.p.n_small;
+    a.p.n_big = it.p.n_big;
     a.vals = it.vals;
     a.K = K;
     a.X = it.buf[step % 3];
@@ -349,10 +356,247 @@ void k_batched_lanes(const BatchItem<T, V>* __restrict__ items, int nitems,
     a.ldz = ld;
     a.W = mode == 1 ? it.v0 : nullptr;
     a.ldw = ld;
+    return a;
+}
+
+// Largest item index whose base (g_base or g_base2) is <= gg, uniform.
+template <typename T, typename V, bool SECOND>
+__device__ __forceinline__ int batch_item(const BatchItem<T, V>* items, int nitems, int gg) {
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if ((SECOND ? items[mid].g_base2 : items[mid].g_base) <= gg) lo = mid;
+        else hi = mid - 1;
+    }
+    return uniform(lo);
+}
+
+// One KPM recursion step over every matrix of a batch (see kpm.hip): work-group
+// gg finds its matrix by binary search over g_base; groups past the last
+// matrix's rows write zero partials.
+template <typename T, typename V, int S, int KT, int VN>
+__global__ __launch_bounds__(kThreads)
+void k_batched_lanes(const BatchItem<T, V>* __restrict__ items, int nitems,
+                                                            int Gtot, int K, int ld, int step, double alpha,
+                                                            double beta, double gamma0, double gamma, int mode,
+                                                            double* __restrict__ partials, int parts) {
+    const int RPG = kWaves * (64 / (S * parts));
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int gg = xcd_chunk(Gtot);
+    const int lo = batch_item<T, V, false>(items, nitems, gg);
+    using CItem = const __attribute__((address_space(4))) BatchItem<T, V>;
+    CItem& it = ((CItem*)items)[lo];
+    ApplyArgs<T, V> a = batch_args(items, lo, K, ld, step, alpha, beta, gamma0, gamma, mode);
+    a.ldw = ld;
+    a.row_list = it.row_list;
+    a.parts = parts;
     const int64_t sb = int64_t(gg - it.g_base) * RPG;
-    const int64_t se = sb + RPG < it.n_rows ? sb + RPG : it.n_rows;
+    const int64_t se = sb + RPG < it.lane_rows ? sb + RPG : it.lane_rows;
     double d1 = 0.0, d2 = 0.0;
     lane_rows<T, V, 0, S, KT, VN, true>(a, sb, se, lane, wave, d1, d2);
+    write_partials(1, lane, wave, d1, d2, partials + int64_t(gg) * 2);
+}
+
+// ------------------------------------------------------------ matrix-core class
+// Block rows of 8..16 rows (real types): one wavefront per block row computes
+// the r x K output as ceil(K/16) tiles of 16 x 16 with v_mfma_f64_16x16x4_f64
+// (A operand: lane l holds row l%16, block column kk + l/16; B operand: lane l
+// holds X row kk + l/16, column l%16; D: register e of lane l holds row
+// 4e + l/16 of column l%16 -- the FP64 layout, unlike the FP32 16x16x4 one
+// with rows 4(l/16) + e).  Each X element of a block is loaded once per block -- not
+// once per row as in the lane family -- and the FP32 values of precision 1/2
+// are widened exactly, so the products accumulate in FP64 as everywhere else.
+// Rows past r and block columns past c enter as zeros (clamped loads +
+// selects, no guarded loads).
+typedef double vbh_d4 __attribute__((ext_vector_type(4)));
+
+// LDS tile of one wavefront: the current slot's A block (r x c values, read
+// as the aligned 16-byte words that cover it) and its c x K X rows (row b at
+// b*K).  Sizes: A <= 16*16 values + 2 words of alignment, X <= 16 rows of up
+// to 16*NT columns.
+template <typename T, typename V, int NT>
+struct MfmaTile {
+    static constexpr int EVA = 16 / int(sizeof(V));              // values per 16-byte word
+    static constexpr int NWA = (256 + 2 * EVA + EVA - 1) / EVA;  // A words
+    static constexpr int EVX = 16 / int(sizeof(T));
+    static constexpr int XS = 16 * NT;                          // X row stride (elements)
+    static constexpr int NWX = 16 * XS / EVX;                   // X words
+    static constexpr int QA = (NWA + 63) / 64;                  // A words per lane
+    static constexpr int QX = (NWX + 63) / 64;                  // X words per lane
+    vbh_w4 a[NWA];
+    vbh_w4 x[NWX];
+};
+
+// Loads slot u's A block and X rows into registers (clamped, unconditional
+// 16-byte loads); the slot's x offset and width come from lanes of mx, mw.
+template <typename T, typename V, int NT>
+__device__ __forceinline__ void mfma_fetch(const ApplyArgs<T, V>& g, int mx, int mw, int u, int64_t vrow, int r,
+                                           int kv, int lane, vbh_w4 (&ra)[MfmaTile<T, V, NT>::QA],
+                                           vbh_w4 (&rx)[MfmaTile<T, V, NT>::QX], int& lead_u, int& c_u) {
+    using Tl = MfmaTile<T, V, NT>;
+    constexpr int EVA = Tl::EVA;
+    const int xo = lane_read(mx, u), w = lane_read(mw, u);
+    c_u = w & 255;
+    const int64_t s0 = vrow + int64_t(r) * (w >> 8);
+    const int64_t sa = s0 & ~int64_t(EVA - 1);
+    lead_u = int(s0 - sa);
+    const int nwa = (lead_u + r * c_u + EVA - 1) / EVA;
+    const vbh_w4* srcA = reinterpret_cast<const vbh_w4*>(g.vals + sa);
+#pragma unroll
+    for (int h = 0; h < Tl::QA; ++h) {
+        const int e = h * 64 + lane;
+        ra[h] = srcA[e < nwa ? e : nwa - 1];
+    }
+    const int nwx = c_u * kv;
+#pragma unroll
+    for (int h = 0; h < Tl::QX; ++h) {
+        int e = h * 64 + lane;
+        e = e < nwx ? e : nwx - 1;
+        const int b = e / kv, cw = e - b * kv;
+        rx[h] = reinterpret_cast<const vbh_w4*>(g.X + (int64_t(xo) + b) * g.ldx)[cw];
+    }
+}
+
+// One block row per wavefront: for each slot the A block and the slot's X
+// rows are staged in the wave's LDS tile (the next slot's tile is fetched
+// into registers while the current one feeds the MFMAs), and the 16 x 16
+// tiles accumulate in registers.
+template <typename T, typename V, int NT, bool DOT>
+__device__ __forceinline__ void mfma_rows(const ApplyArgs<T, V>& g, int64_t q0, int64_t q1, int lane, int wave,
+                                          double& d1, double& d2, MfmaTile<T, V, NT>* tiles) {
+    using Tl = MfmaTile<T, V, NT>;
+    constexpr int EVA = Tl::EVA, EVX = Tl::EVX;
+    const DevPattern& p = g.p;
+    const int li = lane & 15, lk = lane >> 4;
+    const int K = g.K;
+    const int kv = K / EVX;  // 16-byte words per X row (K % EVX == 0, checked on the host)
+    Tl& tile = tiles[wave];
+    for (int64_t q = q0 + wave; q < q1; q += kWaves) {
+        const int i = p.big_brows[q];
+        const int r = p.rbs[i];
+        const int kb = p.row_ptr[i], ke = p.row_ptr[i + 1];
+        const int64_t vrow = p.voff[kb];
+        vbh_d4 acc[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[t] = vbh_d4{0.0, 0.0, 0.0, 0.0};
+        for (int t0 = kb; t0 < ke; t0 += 64) {
+            const int nb = ke - t0 < 64 ? ke - t0 : 64;
+            // the 64 slots' metadata, one per lane; read back with readlane
+            int mx = 0, mw = 0;
+            {
+                const int2 mm = p.meta[t0 + (lane < nb ? lane : nb - 1)];
+                mx = mm.x;
+                mw = mm.y;
+            }
+            // registers holding slot u's tile while slot u-1 is computed
+            vbh_w4 ra[Tl::QA];
+            vbh_w4 rx[Tl::QX];
+            int lead = 0, c = 0;
+            mfma_fetch<T, V, NT>(g, mx, mw, 0, vrow, r, kv, lane, ra, rx, lead, c);
+            for (int u = 0; u < nb; ++u) {
+                // the fetched tile goes to LDS (stores clamped like the loads:
+                // lanes past the tile rewrite its last word with the same value)
+                {
+                    const int nwa = (lead + r * c + EVA - 1) / EVA;
+#pragma unroll
+                    for (int h = 0; h < Tl::QA; ++h) {
+                        const int e = h * 64 + lane;
+                        tile.a[e < nwa ? e : nwa - 1] = ra[h];
+                    }
+                    const int nwx = c * kv;
+#pragma unroll
+                    for (int h = 0; h < Tl::QX; ++h) {
+                        const int e = h * 64 + lane;
+                        tile.x[e < nwx ? e : nwx - 1] = rx[h];
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const int cc = c, ld0 = lead;
+                if (u + 1 < nb) mfma_fetch<T, V, NT>(g, mx, mw, u + 1, vrow, r, kv, lane, ra, rx, lead, c);
+                const V* la = reinterpret_cast<const V*>(tile.a) + ld0;
+                const T* lx = reinterpret_cast<const T*>(tile.x);  // X row b at b*K
+                const int ar = li < r ? li : r - 1;
+                for (int kk = 0; kk < cc; kk += 4) {
+                    const int bc = kk + lk < cc ? kk + lk : cc - 1;
+                    const bool kon = kk + lk < cc;
+                    const double av = widen(la[ar * cc + bc]);
+                    const double a = (li < r && kon) ? av : 0.0;
+#pragma unroll
+                    for (int t = 0; t < NT; ++t) {
+                        const int col = 16 * t + li;
+                        const double xv = widen(lx[bc * K + (col < K ? col : K - 1)]);
+                        const double b = kon && col < K ? xv : 0.0;
+                        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+                    }
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            }
+        }
+        // register e of lane l holds row 4e + l/16 of column 16t + l%16
+        const int64_t row0 = p.roff[i], x0 = p.coff[i];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int col = 16 * t + li;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int a = 4 * e + lk;
+                if (a < r && col < K) {
+                    const int64_t yr = row0 + a, xr = x0 + a;
+                    double y = g.alpha * acc[t][e];
+                    double xo = 0.0;
+                    if (DOT || g.beta != 0.0) xo = widen(g.X[xr * g.ldx + col]);
+                    if (g.beta != 0.0) y += g.beta * xo;
+                    if (g.gamma != 0.0) y += g.gamma * widen(g.Z[yr * g.ldz + col]);
+                    const T ys = narrow_to<T>(y);
+                    g.Y[yr * g.ldy + col] = ys;
+                    if constexpr (DOT) {
+                        const double yw = widen(ys);
+                        d1 += (g.W ? widen(g.W[yr * g.ldw + col]) : yw) * yw;
+                        d2 += xo * yw;
+                    }
+                }
+            }
+        }
+    }
+}
+
+template <typename T, typename V, int NT, bool DOT>
+__global__ __launch_bounds__(kThreads) void k_apply_mfma(ApplyArgs<T, V> g) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int chunk = xcd_chunk(g.G);
+    const int64_t q0 = int64_t(chunk) * kMfmaRowsPerGroup;
+    const int64_t q1 = q0 + kMfmaRowsPerGroup < g.p.n_big ? q0 + kMfmaRowsPerGroup : g.p.n_big;
+    __shared__ MfmaTile<T, V, NT> tiles[kWaves];
+    double d1 = 0.0, d2 = 0.0;
+    mfma_rows<T, V, NT, DOT>(g, q0, q1, lane, wave, d1, d2, tiles);
+    if constexpr (DOT) write_partials(1, lane, wave, d1, d2, g.partials + int64_t(chunk) * 2);
+}
+
+// The matrix-core class of every matrix of a batch in one launch: work-group
+// gg takes kMfmaRowsPerGroup entries of its matrix's big_brows.
+template <typename T, typename V, int NT>
+__global__ __launch_bounds__(kThreads) void k_batched_mfma(const BatchItem<T, V>* __restrict__ items, int nitems,
+                                                           int Gtot, int K, int ld, int step, double alpha,
+                                                           double beta, double gamma0, double gamma, int mode,
+                                                           double* __restrict__ partials) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int gg = xcd_chunk(Gtot);
+    const int lo = batch_item<T, V, true>(items, nitems, gg);
+    using CItem = const __attribute__((address_space(4))) BatchItem<T, V>;
+    CItem& it = ((CItem*)items)[lo];
+    ApplyArgs<T, V> a = batch_args(items, lo, K, ld, step, alpha, beta, gamma0, gamma, mode);
+    const int64_t q0 = int64_t(gg - it.g_base2) * kMfmaRowsPerGroup;
+    const int64_t q1 = q0 + kMfmaRowsPerGroup < a.p.n_big ? q0 + kMfmaRowsPerGroup : a.p.n_big;
+    __shared__ MfmaTile<T, V, NT> tiles[kWaves];
+    double d1 = 0.0, d2 = 0.0;
+    if (it.row_list) mfma_rows<T, V, NT, true>(a, q0, q1, lane, wave, d1, d2, tiles);
     write_partials(1, lane, wave, d1, d2, partials + int64_t(gg) * 2);
 }
 
@@ -434,13 +678,60 @@ void launch_batched_lanes(const BatchItem<T, V>* items, int nitems, int Gtot, in
         with_vector<T, decltype(KT_)::value>(vn, [&](auto VN_) {
             constexpr int S = decltype(S_)::value, KT = decltype(KT_)::value, VN = decltype(VN_)::value;
             hipLaunchKernelGGL((k_batched_lanes<T, V, S, KT, VN>), dim3(Gtot), dim3(kThreads), 0, s, items, nitems, Gtot,
-                               K, ld, step, alpha, beta, gamma0, gamma, mode, partials);
+                               K, ld, step, alpha, beta, gamma0, gamma, mode, partials, sh.P);
         });
     });
     VBH_CHECK_HIP(hipGetLastError());
 }
 
+template <typename T, typename V>
+void launch_batched_mfma(const BatchItem<T, V>* items, int nitems, int Gtot, int K, int ld, int step, double alpha,
+                         double beta, double gamma0, double gamma, int mode, double* partials, hipStream_t s) {
+    if constexpr (Sc<T>::is_complex) {
+        throw Error(VBH_ERR_NOTSUP, "internal: matrix-core class is real-only");
+    } else {
+        auto go = [&](auto NT_) {
+            constexpr int NT = decltype(NT_)::value;
+            hipLaunchKernelGGL((k_batched_mfma<T, V, NT>), dim3(Gtot), dim3(kThreads), 0, s, items, nitems, Gtot, K,
+                               ld, step, alpha, beta, gamma0, gamma, mode, partials);
+        };
+        switch ((K + 15) / 16) {
+            case 1: go(std::integral_constant<int, 1>{}); break;
+            case 2: go(std::integral_constant<int, 2>{}); break;
+            case 3: go(std::integral_constant<int, 3>{}); break;
+            default: go(std::integral_constant<int, 4>{}); break;
+        }
+        VBH_CHECK_HIP(hipGetLastError());
+    }
+}
+
+template <typename T, typename V>
+void launch_mfma(const ApplyArgs<T, V>& a, hipStream_t s) {
+    if constexpr (Sc<T>::is_complex) {
+        throw Error(VBH_ERR_NOTSUP, "internal: matrix-core class is real-only");
+    } else {
+        const int nt = (a.K + 15) / 16;
+        const bool dot = a.partials != nullptr;
+        const dim3 grid(a.G), block(kThreads);
+        auto go = [&](auto NT_) {
+            constexpr int NT = decltype(NT_)::value;
+            if (dot) hipLaunchKernelGGL((k_apply_mfma<T, V, NT, true>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((k_apply_mfma<T, V, NT, false>), grid, block, 0, s, a);
+        };
+        switch (nt) {
+            case 1: go(std::integral_constant<int, 1>{}); break;
+            case 2: go(std::integral_constant<int, 2>{}); break;
+            case 3: go(std::integral_constant<int, 3>{}); break;
+            default: go(std::integral_constant<int, 4>{}); break;
+        }
+        VBH_CHECK_HIP(hipGetLastError());
+    }
+}
+
 #define VBH_LANES_INSTANTIATE(T, V)                                                                             \
+    template void launch_mfma<T, V>(const ApplyArgs<T, V>&, hipStream_t);                                       \
+    template void launch_batched_mfma<T, V>(const BatchItem<T, V>*, int, int, int, int, int, double, double,    \
+                                            double, double, int, double*, hipStream_t);                        \
     template void launch_lanes<T, V>(const ApplyArgs<T, V>&, int64_t, LaneShape, int, hipStream_t);            \
     template void launch_batched_lanes<T, V>(const BatchItem<T, V>*, int, int, int, int, LaneShape, int, double, \
                                              double, double, double, int, double*, hipStream_t);

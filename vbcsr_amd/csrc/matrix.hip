@@ -272,9 +272,22 @@ vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const
         m->meta = upload(meta, s);
         {
             std::vector<int> row_of(std::max<int64_t>(m->n_rows, 1), 0);
-            for (int i = 0; i < nbr; ++i)
-                for (int t = roff[i]; t < roff[i + 1]; ++t) row_of[t] = i;
+            std::vector<int> small, big;
+            for (int i = 0; i < nbr; ++i) {
+                const bool is_big = rbs[i] >= kMfmaMinRows && rbs[i] <= 16;
+                if (is_big) big.push_back(i);
+                for (int t = roff[i]; t < roff[i + 1]; ++t) {
+                    row_of[t] = i;
+                    if (!is_big) small.push_back(t);
+                }
+            }
             m->row_of = upload(row_of, s);
+            if (!big.empty()) {
+                m->n_big = int(big.size());
+                m->n_small = int64_t(small.size());
+                m->big_brows = upload(big, s);
+                m->small_rows = upload(small, s);
+            }
         }
         m->split = upload(split, s);
         // kVecPad trailing elements: the streaming kernels read whole 16-byte
@@ -292,7 +305,8 @@ vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const
 void destroy_matrix(vbh_matrix* m) {
     if (!m) return;
     for (void* p : {(void*)m->rbs, (void*)m->cbs, (void*)m->row_ptr, (void*)m->col_ind, (void*)m->roff,
-                    (void*)m->coff, (void*)m->voff, (void*)m->meta, (void*)m->row_of, m->vals, m->vals32,
+                    (void*)m->coff, (void*)m->voff, (void*)m->meta, (void*)m->row_of, (void*)m->small_rows,
+                    (void*)m->big_brows, m->vals, m->vals32,
                     (void*)m->split, (void*)m->t_ptr,
                     (void*)m->t_row, (void*)m->t_slot, (void*)m->t_split})
         dfree(p);
