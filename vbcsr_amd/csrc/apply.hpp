@@ -271,8 +271,12 @@ __device__ __forceinline__ void spmv_issue(const ApplyArgs<T>& g, int64_t vrow, 
     act = u <= ulast && a_l < r;
 #pragma unroll
     for (int q = 0; q < spmv_words<T, RPP>(); ++q) {
-        const int e = q * 64 + lane;
-        stage[q] = src[e < nvec ? e : nvec - 1];
+        // words past the run are skipped on a wave-uniform condition (a scalar
+        // branch: no per-lane wait); inside the run the index is clamped
+        if (q == 0 || q * 64 < nvec) {
+            const int e = q * 64 + lane;
+            stage[q] = src[e < nvec ? e : nvec - 1];
+        }
     }
     // lane = p*RPP + b loads x[xo_p + b] (clamped into the block's width)
     const int c = w & 255;
@@ -318,8 +322,10 @@ __device__ inline void row_loop_spmv(const ApplyArgs<T>& g, int r0, int r1, int 
                     vbh_w4* dstw = reinterpret_cast<vbh_w4*>(lds);
 #pragma unroll
                     for (int q = 0; q < NV; ++q) {
-                        const int e = q * 64 + lane;
-                        dstw[e < nvec ? e : nvec - 1] = stage[q];
+                        if (q == 0 || q * 64 < nvec) {
+                            const int e = q * 64 + lane;
+                            dstw[e < nvec ? e : nvec - 1] = stage[q];
+                        }
                     }
                 }
                 lds[kSpmvX(RPP) + lane] = xone;
@@ -328,9 +334,11 @@ __device__ inline void row_loop_spmv(const ApplyArgs<T>& g, int r0, int r1, int 
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#ifndef VBH_SPMV_NOPIPE
                 if (ub + CH < nb)  // overlaps the FMAs below
                     spmv_issue<T, RPP>(g, vrow, r, nb, ub + CH, mx, mw, lane, stage, xone, lead, nvec, pc_first, w,
                                        act);
+#endif
                 if (act_c) {
                     const int c = w_c & 255;
                     const int cm = c > 0 ? c - 1 : 0;
@@ -345,6 +353,11 @@ __device__ inline void row_loop_spmv(const ApplyArgs<T>& g, int r0, int r1, int 
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#ifdef VBH_SPMV_NOPIPE
+                if (ub + CH < nb)
+                    spmv_issue<T, RPP>(g, vrow, r, nb, ub + CH, mx, mw, lane, stage, xone, lead, nvec, pc_first, w,
+                                       act);
+#endif
             }
         }
 #pragma unroll

@@ -249,7 +249,10 @@ vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const
         m->n_classes = int(std::count(cls.begin(), cls.end(), 1));
         m->nnz = m->h_voff[m->nnzb];
         // work-balanced chunks of rows, G a multiple of 8 (one run per XCD)
-        int G = int(std::min<int64_t>(std::max<int64_t>((nbr + kWaves - 1) / kWaves, 8), 4096));
+#ifndef VBH_MAX_GROUPS
+#define VBH_MAX_GROUPS 8192
+#endif
+        int G = int(std::min<int64_t>(std::max<int64_t>((nbr + kWaves - 1) / kWaves, 8), VBH_MAX_GROUPS));
         G = (G + 7) & ~7;
         std::vector<int> split(G + 1, nbr);
         split[0] = 0;
