@@ -32,6 +32,10 @@ HIP_FLAGS = [
     "-mcode-object-version=5",
     # no packed-FP32 instructions in device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # MFMA accumulators in VGPRs: the matrix-core row loop branches on the
+    # block width between MFMAs, and AGPR accumulators then cost a copy to
+    # and from VGPRs around every k-step
+    "-mllvm", "-amdgpu-mfma-vgpr-form",
     "-Wno-unused-result",
 ]
 

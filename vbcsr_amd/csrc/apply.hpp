@@ -95,6 +95,12 @@ __device__ inline T narrow_to(typename Wide<T>::type x) {
 }
 
 __device__ inline int lane_read(int v, int lane_idx) { return __builtin_amdgcn_readlane(v, lane_idx); }
+// A 64-bit value the wave holds in every lane, as a scalar.
+__device__ inline int64_t uniform64(int64_t v) {
+    const uint64_t u = uint64_t(v);
+    const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(u)), hi = __builtin_amdgcn_readfirstlane(uint32_t(u >> 32));
+    return int64_t((uint64_t(hi) << 32) | lo);
+}
 
 // Fused epilogue for one output element (row yr, column v).
 template <typename T, bool FUSED>
@@ -480,17 +486,9 @@ inline int mfma_groups(int n_big) {
     const int g = (n_big + kMfmaRowsPerGroup - 1) / kMfmaRowsPerGroup;
     return ((g < 8 ? 8 : g) + 7) & ~7;
 }
-// The LDS tile holds blocks up to 16 x 16 and reads X rows as aligned 16-byte
-// words (K a multiple of 16/sizeof(T), X and ldx aligned to match).
-template <typename T>
-inline bool mfma_x_ok(const void* X, int64_t ldx, int K) {
-    const int ev = int(16 / sizeof(T));
-    return K % ev == 0 && ldx % ev == 0 && reinterpret_cast<uintptr_t>(X) % 16 == 0;
-}
 template <typename T, typename V>
 inline bool uses_mfma(const vbh_matrix* m, const ApplyArgs<T, V>& a) {
-    return !Sc<T>::is_complex && m->n_big > 0 && m->max_block <= 16 && uses_lanes(a) &&
-           mfma_x_ok<T>(a.X, a.ldx, a.K);
+    return !Sc<T>::is_complex && m->n_big > 0 && m->max_block <= 16 && uses_lanes(a);
 }
 // Work-groups of the lane part (over the small rows when the matrix-core
 // class runs apart) and of the matrix-core part.

@@ -192,7 +192,10 @@ struct DevPattern {
 };
 
 constexpr int kWaves = 4;  // waves per work-group in the row kernels
-constexpr int kMfmaMinRows = 8;  // block rows of 8..16 rows run on 16x16x4 FP64 matrix-core tiles
+#ifndef VBH_MFMA_MIN_ROWS
+#define VBH_MFMA_MIN_ROWS 4
+#endif
+constexpr int kMfmaMinRows = VBH_MFMA_MIN_ROWS;  // block rows of kMfmaMinRows..16 rows run on 16x16x4 FP64 matrix-core tiles
 constexpr int kThreads = 64 * kWaves;
 constexpr int kVecPad = 8;  // padding elements after every value array
 

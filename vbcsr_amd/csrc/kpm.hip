@@ -161,6 +161,19 @@ struct StepTimer {
     }
 };
 
+// A second stream (default priority) and two events for fork/join of
+// independent launches, eager or under graph capture.
+struct StreamFork {
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    void init() {
+        if (side) return;
+        VBH_CHECK_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+        VBH_CHECK_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        VBH_CHECK_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+    }
+};
+
 template <typename T>
 static int64_t step_count(int M, int mode) {
     if (M <= 1) return 0;
@@ -328,7 +341,7 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
     // loop over each matrix's work-balanced split.  Dots are summed over the
     // columns (E = 2) everywhere.
     LaneShape sh = lane_shape_of<T, V>(K);
-    bool mfma_ok = !Sc<T>::is_complex && sh.S > 0 && mfma_x_ok<T>(nullptr, K, K);  // buffers: ld = K, 256-B aligned
+    bool mfma_ok = !Sc<T>::is_complex && sh.S > 0;
     for (int h = 0; h < count; ++h) mfma_ok = mfma_ok && ms[h]->max_block <= 16;
     if (sh.S) {  // slot partitions from the batch's total lane rows
         int64_t lrows = 0;
@@ -397,16 +410,30 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
         int max_block = 1;
         for (int h = 0; h < count; ++h) max_block = std::max(max_block, ms[h]->max_block);
         const int rb = row_config(K < 2 ? 2 : K, max_block, 0).rb;
+        static StreamFork fork;
+        if (sh.S && Gtot2 > 0) fork.init();
         auto launch_all = [&](hipStream_t st) {
             for (int64_t step = 0; step < nsteps; ++step) {
                 const Item* it = ditems;
                 const double al = step == 0 ? a1 : a2, be = step == 0 ? b1 : b2;
                 double* pt = partials + step * Gtot * E;
                 if (sh.S) {
-                    launch_batched_lanes<T, V>(it, count, Gtot1, K, K, sh, int(step), al, be, 0.0, -1.0, mode, pt, st);
-                    if (Gtot2 > 0)
+                    if (Gtot2 > 0) {
+                        // the matrix-core launch of a step is independent of
+                        // its lane launch: forked onto the side stream and
+                        // joined back, so the two run side by side
+                        VBH_CHECK_HIP(hipEventRecord(fork.fork, st));
+                        VBH_CHECK_HIP(hipStreamWaitEvent(fork.side, fork.fork, 0));
                         launch_batched_mfma<T, V>(it, count, Gtot2, K, K, int(step), al, be, 0.0, -1.0, mode,
-                                                  pt + int64_t(Gtot1) * E, st);
+                                                  pt + int64_t(Gtot1) * E, fork.side);
+                    }
+                    if (Gtot1 > 0)
+                        launch_batched_lanes<T, V>(it, count, Gtot1, K, K, sh, int(step), al, be, 0.0, -1.0, mode, pt,
+                                                   st);
+                    if (Gtot2 > 0) {
+                        VBH_CHECK_HIP(hipEventRecord(fork.join, fork.side));
+                        VBH_CHECK_HIP(hipStreamWaitEvent(st, fork.join, 0));
+                    }
                     continue;
                 }
                 if constexpr (!is_mixed<T, V>()) {  // K == 1, single type: row loop

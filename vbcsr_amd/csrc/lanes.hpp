@@ -399,84 +399,83 @@ void k_batched_lanes(const BatchItem<T, V>* __restrict__ items, int nitems,
 }
 
 // ------------------------------------------------------------ matrix-core class
-// Block rows of 8..16 rows (real types): one wavefront per block row computes
+// Block rows of kMfmaMinRows..16 rows (real types): one wavefront per block row computes
 // the r x K output as ceil(K/16) tiles of 16 x 16 with v_mfma_f64_16x16x4_f64
 // (A operand: lane l holds row l%16, block column kk + l/16; B operand: lane l
 // holds X row kk + l/16, column l%16; D: register e of lane l holds row
 // 4e + l/16 of column l%16 -- the FP64 layout, unlike the FP32 16x16x4 one
-// with rows 4(l/16) + e).  Each X element of a block is loaded once per block -- not
-// once per row as in the lane family -- and the FP32 values of precision 1/2
-// are widened exactly, so the products accumulate in FP64 as everywhere else.
-// Rows past r and block columns past c enter as zeros (clamped loads +
-// selects, no guarded loads).
+// with rows 4(l/16) + e).  Each X element of a block is loaded once per block
+// -- not once per row as in the lane family -- and the FP32 values of
+// precision 1/2 are widened exactly, so the products accumulate in FP64 as
+// everywhere else.  Rows past r and block columns past c enter as zeros
+// (clamped loads + selects, no guarded loads).
 typedef double vbh_d4 __attribute__((ext_vector_type(4)));
 
-// LDS tile of one wavefront: the current slot's A block (r x c values, read
-// as the aligned 16-byte words that cover it) and its c x K X rows (row b at
-// b*K).  Sizes: A <= 16*16 values + 2 words of alignment, X <= 16 rows of up
-// to 16*NT columns.
+// Operands of slot u of the current 64-slot chunk, loaded straight into the
+// MFMA layout: a[s] = A[li][4s + lk], x[s][t] = X[xo + 4s + lk][16t + li],
+// for the 4 k-steps that cover a block of up to 16 columns.  Indices are
+// clamped into the block and into K, so every load is unconditional and in
+// bounds and the number of loads per slot is fixed (the compiler then waits
+// for exactly one slot's loads while the next slot's are in flight).  The
+// slot's x offset and width come from lanes of mx, mw.
 template <typename T, typename V, int NT>
-struct MfmaTile {
-    static constexpr int EVA = 16 / int(sizeof(V));              // values per 16-byte word
-    static constexpr int NWA = (256 + 2 * EVA + EVA - 1) / EVA;  // A words
-    static constexpr int EVX = 16 / int(sizeof(T));
-    static constexpr int XS = 16 * NT;                          // X row stride (elements)
-    static constexpr int NWX = 16 * XS / EVX;                   // X words
-    static constexpr int QA = (NWA + 63) / 64;                  // A words per lane
-    static constexpr int QX = (NWX + 63) / 64;                  // X words per lane
-    vbh_w4 a[NWA];
-    vbh_w4 x[NWX];
-};
-
-// Loads slot u's A block and X rows into registers (clamped, unconditional
-// 16-byte loads); the slot's x offset and width come from lanes of mx, mw.
-template <typename T, typename V, int NT>
-__device__ __forceinline__ void mfma_fetch(const ApplyArgs<T, V>& g, int mx, int mw, int u, int64_t vrow, int r,
-                                           int kv, int lane, vbh_w4 (&ra)[MfmaTile<T, V, NT>::QA],
-                                           vbh_w4 (&rx)[MfmaTile<T, V, NT>::QX], int& lead_u, int& c_u) {
-    using Tl = MfmaTile<T, V, NT>;
-    constexpr int EVA = Tl::EVA;
+__device__ __forceinline__ void mfma_load(const ApplyArgs<T, V>& g, int mx, int mw, int u, int64_t vrow, int r,
+                                          int li, int lk, V (&a)[4], T (&x)[4][NT], int& c_u) {
     const int xo = lane_read(mx, u), w = lane_read(mw, u);
-    c_u = w & 255;
-    const int64_t s0 = vrow + int64_t(r) * (w >> 8);
-    const int64_t sa = s0 & ~int64_t(EVA - 1);
-    lead_u = int(s0 - sa);
-    const int nwa = (lead_u + r * c_u + EVA - 1) / EVA;
-    const vbh_w4* srcA = reinterpret_cast<const vbh_w4*>(g.vals + sa);
+    const int c = w & 255;
+    __builtin_assume(c >= 1);
+    c_u = c;
+    const V* ab = g.vals + vrow + int64_t(r) * (w >> 8) + (li < r ? li : r - 1) * c;
 #pragma unroll
-    for (int h = 0; h < Tl::QA; ++h) {
-        const int e = h * 64 + lane;
-        ra[h] = srcA[e < nwa ? e : nwa - 1];
-    }
-    const int nwx = c_u * kv;
+    for (int s = 0; s < 4; ++s) {
+        const int b = 4 * s + lk;
+        const int bc = b < c ? b : c - 1;
+        a[s] = ab[bc];
+        const T* xr = g.X + int64_t(xo + bc) * g.ldx;
 #pragma unroll
-    for (int h = 0; h < Tl::QX; ++h) {
-        int e = h * 64 + lane;
-        e = e < nwx ? e : nwx - 1;
-        const int b = e / kv, cw = e - b * kv;
-        rx[h] = reinterpret_cast<const vbh_w4*>(g.X + (int64_t(xo) + b) * g.ldx)[cw];
+        for (int t = 0; t < NT; ++t) {
+            const int col = 16 * t + li;
+            x[s][t] = xr[col < g.K ? col : g.K - 1];
+        }
     }
 }
 
-// One block row per wavefront: for each slot the A block and the slot's X
-// rows are staged in the wave's LDS tile (the next slot's tile is fetched
-// into registers while the current one feeds the MFMAs), and the 16 x 16
-// tiles accumulate in registers.
+// The MFMAs of one slot: ceil(c/4) k-steps (a wave-uniform count) of NT tiles.
+template <typename T, typename V, int NT>
+__device__ __forceinline__ void mfma_slot(int c, int r, int li, int lk, int K, const V (&a)[4], const T (&x)[4][NT],
+                                          vbh_d4 (&acc)[NT]) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+        if (4 * s < c) {
+            const bool kon = 4 * s + lk < c;
+            const double av = (li < r && kon) ? widen(a[s]) : 0.0;
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const double bv = (kon && 16 * t + li < K) ? widen(x[s][t]) : 0.0;
+                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[t], 0, 0, 0);
+            }
+        }
+    }
+}
+
+// One block row per wavefront, entries [q0, q1) of big_brows wave-
+// interleaved.  Two slots are in flight: while slot u feeds the MFMAs, the
+// operands of slots u+1 and u+2 are being loaded (past the chunk's last slot
+// the loads repeat it; those products are never issued).
 template <typename T, typename V, int NT, bool DOT>
 __device__ __forceinline__ void mfma_rows(const ApplyArgs<T, V>& g, int64_t q0, int64_t q1, int lane, int wave,
-                                          double& d1, double& d2, MfmaTile<T, V, NT>* tiles) {
-    using Tl = MfmaTile<T, V, NT>;
-    constexpr int EVA = Tl::EVA, EVX = Tl::EVX;
+                                          double& d1, double& d2) {
     const DevPattern& p = g.p;
     const int li = lane & 15, lk = lane >> 4;
     const int K = g.K;
-    const int kv = K / EVX;  // 16-byte words per X row (K % EVX == 0, checked on the host)
-    Tl& tile = tiles[wave];
-    for (int64_t q = q0 + wave; q < q1; q += kWaves) {
-        const int i = p.big_brows[q];
-        const int r = p.rbs[i];
-        const int kb = p.row_ptr[i], ke = p.row_ptr[i + 1];
-        const int64_t vrow = p.voff[kb];
+    // the row header is wave-uniform: readfirstlane keeps the slot loop a
+    // scalar loop (the compiler cannot see that the wave index is uniform)
+    for (int64_t q = q0 + __builtin_amdgcn_readfirstlane(wave); q < q1; q += kWaves) {
+        const int i = __builtin_amdgcn_readfirstlane(p.big_brows[q]);
+        const int r = __builtin_amdgcn_readfirstlane(p.rbs[i]);
+        const int kb = __builtin_amdgcn_readfirstlane(p.row_ptr[i]);
+        const int ke = __builtin_amdgcn_readfirstlane(p.row_ptr[i + 1]);
+        const int64_t vrow = uniform64(p.voff[kb]);
         vbh_d4 acc[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[t] = vbh_d4{0.0, 0.0, 0.0, 0.0};
@@ -489,76 +488,53 @@ __device__ __forceinline__ void mfma_rows(const ApplyArgs<T, V>& g, int64_t q0, 
                 mx = mm.x;
                 mw = mm.y;
             }
-            // registers holding slot u's tile while slot u-1 is computed
-            vbh_w4 ra[Tl::QA];
-            vbh_w4 rx[Tl::QX];
-            int lead = 0, c = 0;
-            mfma_fetch<T, V, NT>(g, mx, mw, 0, vrow, r, kv, lane, ra, rx, lead, c);
-            for (int u = 0; u < nb; ++u) {
-                // the fetched tile goes to LDS (stores clamped like the loads:
-                // lanes past the tile rewrite its last word with the same value)
-                {
-                    const int nwa = (lead + r * c + EVA - 1) / EVA;
-#pragma unroll
-                    for (int h = 0; h < Tl::QA; ++h) {
-                        const int e = h * 64 + lane;
-                        tile.a[e < nwa ? e : nwa - 1] = ra[h];
-                    }
-                    const int nwx = c * kv;
-#pragma unroll
-                    for (int h = 0; h < Tl::QX; ++h) {
-                        const int e = h * 64 + lane;
-                        tile.x[e < nwx ? e : nwx - 1] = rx[h];
-                    }
-                }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                const int cc = c, ld0 = lead;
-                if (u + 1 < nb) mfma_fetch<T, V, NT>(g, mx, mw, u + 1, vrow, r, kv, lane, ra, rx, lead, c);
-                const V* la = reinterpret_cast<const V*>(tile.a) + ld0;
-                const T* lx = reinterpret_cast<const T*>(tile.x);  // X row b at b*K
-                const int ar = li < r ? li : r - 1;
-                for (int kk = 0; kk < cc; kk += 4) {
-                    const int bc = kk + lk < cc ? kk + lk : cc - 1;
-                    const bool kon = kk + lk < cc;
-                    const double av = widen(la[ar * cc + bc]);
-                    const double a = (li < r && kon) ? av : 0.0;
-#pragma unroll
-                    for (int t = 0; t < NT; ++t) {
-                        const int col = 16 * t + li;
-                        const double xv = widen(lx[bc * K + (col < K ? col : K - 1)]);
-                        const double b = kon && col < K ? xv : 0.0;
-                        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
-                    }
-                }
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            V a0[4], a1[4];
+            T x0[4][NT], x1[4][NT];
+            int c0 = 0, c1 = 0;
+            mfma_load<T, V, NT>(g, mx, mw, 0, vrow, r, li, lk, a0, x0, c0);
+            mfma_load<T, V, NT>(g, mx, mw, nb > 1 ? 1 : 0, vrow, r, li, lk, a1, x1, c1);
+            for (int u = 0; u < nb; u += 2) {
+                mfma_slot<T, V, NT>(c0, r, li, lk, K, a0, x0, acc);
+                mfma_load<T, V, NT>(g, mx, mw, u + 2 < nb ? u + 2 : nb - 1, vrow, r, li, lk, a0, x0, c0);
+                if (u + 1 < nb) mfma_slot<T, V, NT>(c1, r, li, lk, K, a1, x1, acc);
+                mfma_load<T, V, NT>(g, mx, mw, u + 3 < nb ? u + 3 : nb - 1, vrow, r, li, lk, a1, x1, c1);
             }
         }
-        // register e of lane l holds row 4e + l/16 of column 16t + l%16
-        const int64_t row0 = p.roff[i], x0 = p.coff[i];
+        // register e of lane l holds row 4e + l/16 of column 16t + l%16.  All
+        // epilogue loads (clamped rows and columns) are issued before the
+        // first use, so the row costs one more latency, not one per element
+        const int64_t row0 = p.roff[i], x0r = p.coff[i];
+        const bool need_x = DOT || g.beta != 0.0, need_z = g.gamma != 0.0, need_w = DOT && g.W != nullptr;
+        double xo[NT][4], zo[NT][4], wo[NT][4];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const int col = 16 * t + li;
+            const int cc = col < K ? col : K - 1;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int a = 4 * e + lk;
+                const int ac = a < r ? a : r - 1;
+                xo[t][e] = need_x ? widen(g.X[(x0r + ac) * g.ldx + cc]) : 0.0;
+                zo[t][e] = need_z ? widen(g.Z[(row0 + ac) * g.ldz + cc]) : 0.0;
+                wo[t][e] = need_w ? widen(g.W[(row0 + ac) * g.ldw + cc]) : 0.0;
+            }
+        }
 #pragma unroll
         for (int t = 0; t < NT; ++t) {
             const int col = 16 * t + li;
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
                 const int a = 4 * e + lk;
-                if (a < r && col < K) {
-                    const int64_t yr = row0 + a, xr = x0 + a;
-                    double y = g.alpha * acc[t][e];
-                    double xo = 0.0;
-                    if (DOT || g.beta != 0.0) xo = widen(g.X[xr * g.ldx + col]);
-                    if (g.beta != 0.0) y += g.beta * xo;
-                    if (g.gamma != 0.0) y += g.gamma * widen(g.Z[yr * g.ldz + col]);
-                    const T ys = narrow_to<T>(y);
-                    g.Y[yr * g.ldy + col] = ys;
-                    if constexpr (DOT) {
-                        const double yw = widen(ys);
-                        d1 += (g.W ? widen(g.W[yr * g.ldw + col]) : yw) * yw;
-                        d2 += xo * yw;
-                    }
+                const bool on = a < r && col < K;
+                double y = g.alpha * acc[t][e];
+                if (g.beta != 0.0) y += g.beta * xo[t][e];
+                if (need_z) y += g.gamma * zo[t][e];
+                const T ys = narrow_to<T>(y);
+                if (on) g.Y[(row0 + a) * g.ldy + col] = ys;
+                if constexpr (DOT) {
+                    const double yw = widen(ys);
+                    d1 += on ? (need_w ? wo[t][e] : yw) * yw : 0.0;
+                    d2 += on ? xo[t][e] * yw : 0.0;
                 }
             }
         }
@@ -572,9 +548,8 @@ __global__ __launch_bounds__(kThreads) void k_apply_mfma(ApplyArgs<T, V> g) {
     const int chunk = xcd_chunk(g.G);
     const int64_t q0 = int64_t(chunk) * kMfmaRowsPerGroup;
     const int64_t q1 = q0 + kMfmaRowsPerGroup < g.p.n_big ? q0 + kMfmaRowsPerGroup : g.p.n_big;
-    __shared__ MfmaTile<T, V, NT> tiles[kWaves];
     double d1 = 0.0, d2 = 0.0;
-    mfma_rows<T, V, NT, DOT>(g, q0, q1, lane, wave, d1, d2, tiles);
+    mfma_rows<T, V, NT, DOT>(g, q0, q1, lane, wave, d1, d2);
     if constexpr (DOT) write_partials(1, lane, wave, d1, d2, g.partials + int64_t(chunk) * 2);
 }
 
@@ -594,9 +569,8 @@ __global__ __launch_bounds__(kThreads) void k_batched_mfma(const BatchItem<T, V>
     ApplyArgs<T, V> a = batch_args(items, lo, K, ld, step, alpha, beta, gamma0, gamma, mode);
     const int64_t q0 = int64_t(gg - it.g_base2) * kMfmaRowsPerGroup;
     const int64_t q1 = q0 + kMfmaRowsPerGroup < a.p.n_big ? q0 + kMfmaRowsPerGroup : a.p.n_big;
-    __shared__ MfmaTile<T, V, NT> tiles[kWaves];
     double d1 = 0.0, d2 = 0.0;
-    if (it.row_list) mfma_rows<T, V, NT, true>(a, q0, q1, lane, wave, d1, d2, tiles);
+    if (it.row_list) mfma_rows<T, V, NT, true>(a, q0, q1, lane, wave, d1, d2);
     write_partials(1, lane, wave, d1, d2, partials + int64_t(gg) * 2);
 }
 

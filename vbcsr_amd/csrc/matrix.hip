@@ -185,6 +185,32 @@ static T scalar_from(const double* a) {
     return Sc<T>::make(a[0], a[1]);
 }
 
+// Breadth-first order of block rows over the local square part of the
+// pattern (columns < nbr), restarted at the lowest unvisited row for each
+// connected component.
+static std::vector<int> bfs_order(int nbr, const int* row_ptr, const int* col_ind) {
+    std::vector<int> order;
+    order.reserve(nbr);
+    std::vector<char> seen(nbr, 0);
+    for (int start = 0; start < nbr; ++start) {
+        if (seen[start]) continue;
+        seen[start] = 1;
+        size_t head = order.size();
+        order.push_back(start);
+        while (head < order.size()) {
+            const int i = order[head++];
+            for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+                const int j = col_ind[k];
+                if (j < nbr && !seen[j]) {
+                    seen[j] = 1;
+                    order.push_back(j);
+                }
+            }
+        }
+    }
+    return order;
+}
+
 vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const int* row_ptr, const int* col_ind,
                          int dtype, hipStream_t s) {
     VBH_REQUIRE(nbr >= 0 && nbc >= 0, "negative block counts");
@@ -275,16 +301,30 @@ vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const
         m->meta = upload(meta, s);
         {
             std::vector<int> row_of(std::max<int64_t>(m->n_rows, 1), 0);
+            for (int i = 0; i < nbr; ++i)
+                for (int t = roff[i]; t < roff[i + 1]; ++t) row_of[t] = i;
+            m->row_of = upload(row_of, s);
+            // matrix-core class: block rows of kMfmaMinRows..16 rows of a
+            // variable-block matrix (uniform 4 x 4 matrices keep the
+            // quad-shared lane path)
+            auto is_big = [&](int i) {
+                return m->uniform_bs != 4 && rbs[i] >= kMfmaMinRows && rbs[i] <= 16;
+            };
+            bool any_big = false;
+            for (int i = 0; i < nbr && !any_big; ++i) any_big = is_big(i);
             std::vector<int> small, big;
-            for (int i = 0; i < nbr; ++i) {
-                const bool is_big = rbs[i] >= kMfmaMinRows && rbs[i] <= 16;
-                if (is_big) big.push_back(i);
-                for (int t = roff[i]; t < roff[i + 1]; ++t) {
-                    row_of[t] = i;
-                    if (!is_big) small.push_back(t);
+            if (any_big) {
+                // Both row lists follow a breadth-first order of the block
+                // graph: rows that run at the same time on one XCD are then
+                // graph neighbours, so the X rows they gather overlap and stay
+                // in that XCD's L2.  Each row is computed exactly as in index
+                // order; only the order of the rows changes.
+                for (int i : bfs_order(nbr, row_ptr, col_ind)) {
+                    if (is_big(i)) big.push_back(i);
+                    else
+                        for (int t = roff[i]; t < roff[i + 1]; ++t) small.push_back(t);
                 }
             }
-            m->row_of = upload(row_of, s);
             if (!big.empty()) {
                 m->n_big = int(big.size());
                 m->n_small = int64_t(small.size());
