@@ -481,7 +481,7 @@ inline bool uses_lanes(const ApplyArgs<T, V>& a) {
 }
 // Matrix-core class: block rows with kMfmaMinRows..16 rows of a real matrix
 // run as 16x16 FP64 MFMA tiles, the other rows on the lane family.
-constexpr int kMfmaRowsPerGroup = kWaves;  // block rows per work-group (one per wave)
+constexpr int kMfmaRowsPerGroup = kWaves * kMfmaRowsPerWave;  // block rows per work-group
 inline int mfma_groups(int n_big) {
     const int g = (n_big + kMfmaRowsPerGroup - 1) / kMfmaRowsPerGroup;
     return ((g < 8 ? 8 : g) + 7) & ~7;

@@ -183,15 +183,19 @@ struct DevPattern {
     const int2* meta;
     const int* row_of;  // n_rows: block row of every scalar row
     // row classes of the real-type SpMM: block rows with kMfmaMinRows <= r <= 16
-    // (matrix-core tiles, big_brows) and the scalar rows of all other block
-    // rows (lane family, small_rows); n_big == 0 when there are none
+    // (matrix-core tiles, big_desc) and the scalar rows of all other block
+    // rows (lane family, small_rows); n_big == 0 when there are none.  Each
+    // matrix-core row has a 32-byte descriptor, so one load yields all it
+    // needs: [0] = {row_ptr[i], row_ptr[i+1], rbs[i], roff[i]},
+    // [1] = {coff[i], voff[row_ptr[i]] (low, high word), i}
     const int* small_rows;
-    const int* big_brows;
+    const int4* big_desc;
     int64_t n_small;
     int n_big;
 };
 
 constexpr int kWaves = 4;  // waves per work-group in the row kernels
+constexpr int kMfmaRowsPerWave = 1;  // matrix-core block rows per wave (the next row's header is prefetched)
 #ifndef VBH_MFMA_MIN_ROWS
 #define VBH_MFMA_MIN_ROWS 4
 #endif
@@ -215,7 +219,8 @@ struct vbh_matrix {
     int64_t* voff = nullptr;
     int2* meta = nullptr;
     int* row_of = nullptr;
-    int *small_rows = nullptr, *big_brows = nullptr;
+    int* small_rows = nullptr;
+    int4* big_desc = nullptr;
     int64_t n_small = 0;
     int n_big = 0;
     // values
@@ -232,7 +237,7 @@ struct vbh_matrix {
 
     vbh::DevPattern pattern() const {
         return vbh::DevPattern{nbr,  nbc,    uniform_bs, rbs,        cbs,       row_ptr, col_ind, roff,
-                               coff, voff,   meta,       row_of,     small_rows, big_brows, n_small, n_big};
+                               coff, voff,   meta,       row_of,     small_rows, big_desc, n_small, n_big};
     }
     size_t elem_size() const { return dtype == VBH_C128 ? 16 : 8; }
 };

@@ -340,7 +340,7 @@ __device__ __forceinline__ ApplyArgs<T, V> batch_args(const BatchItem<T, V>* ite
     a.p.meta = it.p.meta;
     a.p.row_of = it.p.row_of;
     a.p.small_rows = it.p.small_rows;
-    a.p.big_brows = it.p.big_brows;
+    a.p.big_desc = it.p.big_desc;
     a.p.n_small = it.p.n_small;
     a.p.n_big = it.p.n_big;
     a.vals = it.vals;
@@ -411,137 +411,214 @@ void k_batched_lanes(const BatchItem<T, V>* __restrict__ items, int nitems,
 // (clamped loads + selects, no guarded loads).
 typedef double vbh_d4 __attribute__((ext_vector_type(4)));
 
+// Operand layout.  The k index of the MFMAs runs over a block's columns in
+// the order b(s, lk) = 8*(s/2) + 2*lk + s%2 (k-step s, lane group lk), so a
+// lane's A values of k-steps 2m and 2m+1 are adjacent, A[li][8m + 2lk + {0,1}]
+// (one 2-wide load per pair of k-steps); a block of c columns needs k-steps
+// s with 8*(s/2) + s%2 < c.  The output columns are interleaved: tile t,
+// column j is X column NT*j + t, so lane j's NT columns of an X row are
+// adjacent (one NT-wide load per k-step, NT-wide stores in the epilogue).
+// VX: K % NT == 0, the X rows are read with NT-wide loads; otherwise column
+// by column.
+__device__ __forceinline__ int mfma_col(int s, int lk) { return 8 * (s >> 1) + 2 * lk + (s & 1); }
+
+template <typename T, int N>
+__device__ __forceinline__ void load_n(const T* src, T (&out)[N]) {
+    typedef T vec __attribute__((ext_vector_type(N)));
+    vec v;
+    __builtin_memcpy(&v, src, sizeof(vec));  // element-aligned: the target reads it unaligned
+#pragma unroll
+    for (int e = 0; e < N; ++e) out[e] = v[e];
+}
+template <typename T, int N>
+__device__ __forceinline__ void store_n(T* dst, const T (&in)[N]) {
+    typedef T vec __attribute__((ext_vector_type(N)));
+    vec v;
+#pragma unroll
+    for (int e = 0; e < N; ++e) v[e] = in[e];
+    __builtin_memcpy(dst, &v, sizeof(vec));
+}
+
 // Operands of slot u of the current 64-slot chunk, loaded straight into the
-// MFMA layout: a[s] = A[li][4s + lk], x[s][t] = X[xo + 4s + lk][16t + li],
-// for the 4 k-steps that cover a block of up to 16 columns.  Indices are
-// clamped into the block and into K, so every load is unconditional and in
-// bounds and the number of loads per slot is fixed (the compiler then waits
-// for exactly one slot's loads while the next slot's are in flight).  The
-// slot's x offset and width come from lanes of mx, mw.
-template <typename T, typename V, int NT>
+// MFMA layout: a[m][e] = A[li][8m + 2lk + e], x[s][t] = X[xo + b(s, lk)][NT*li + t].
+// Indices are clamped into the block and into K, so every load is
+// unconditional and in bounds (a pair that starts on a row's last value reads
+// one value past it, still inside the value array and its padding), and the
+// number of loads per slot is fixed (the compiler then waits for exactly one
+// slot's loads while the next slot's are in flight).  The slot's x offset and
+// width come from lanes of mx, mw.
+template <typename T, typename V, int NT, bool VX>
 __device__ __forceinline__ void mfma_load(const ApplyArgs<T, V>& g, int mx, int mw, int u, int64_t vrow, int r,
-                                          int li, int lk, V (&a)[4], T (&x)[4][NT], int& c_u) {
+                                          int li, int lk, V (&a)[2][2], T (&x)[4][NT], int& c_u) {
     const int xo = lane_read(mx, u), w = lane_read(mw, u);
     const int c = w & 255;
     __builtin_assume(c >= 1);
     c_u = c;
     const V* ab = g.vals + vrow + int64_t(r) * (w >> 8) + (li < r ? li : r - 1) * c;
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
-        const int b = 4 * s + lk;
-        const int bc = b < c ? b : c - 1;
-        a[s] = ab[bc];
-        const T* xr = g.X + int64_t(xo + bc) * g.ldx;
+    for (int m = 0; m < 2; ++m) {
+        const int p0 = 8 * m + 2 * lk;
+        load_n<V, 2>(ab + (p0 < c ? p0 : c - 1), a[m]);
+    }
+    const int K = g.K;
+    const int c0 = NT * li < K - NT ? NT * li : K - NT;  // VX: lanes past K read the last NT columns
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int col = 16 * t + li;
-            x[s][t] = xr[col < g.K ? col : g.K - 1];
+    for (int s = 0; s < 4; ++s) {
+        const int b = mfma_col(s, lk);
+        const T* xr = g.X + int64_t(xo + (b < c ? b : c - 1)) * g.ldx;
+        if constexpr (VX) {
+            load_n<T, NT>(xr + c0, x[s]);
+        } else {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int col = NT * li + t;
+                x[s][t] = xr[col < K ? col : K - 1];
+            }
         }
     }
 }
 
-// The MFMAs of one slot: ceil(c/4) k-steps (a wave-uniform count) of NT tiles.
+// The MFMAs of one slot: the k-steps its width c needs (a wave-uniform test).
 template <typename T, typename V, int NT>
-__device__ __forceinline__ void mfma_slot(int c, int r, int li, int lk, int K, const V (&a)[4], const T (&x)[4][NT],
-                                          vbh_d4 (&acc)[NT]) {
+__device__ __forceinline__ void mfma_slot(int c, int r, int li, int lk, int K, const V (&a)[2][2],
+                                          const T (&x)[4][NT], vbh_d4 (&acc)[NT]) {
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
-        if (4 * s < c) {
-            const bool kon = 4 * s + lk < c;
-            const double av = (li < r && kon) ? widen(a[s]) : 0.0;
+        if (8 * (s >> 1) + (s & 1) < c) {
+            const bool kon = mfma_col(s, lk) < c;
+            const double av = (li < r && kon) ? widen(a[s >> 1][s & 1]) : 0.0;
 #pragma unroll
             for (int t = 0; t < NT; ++t) {
-                const double bv = (kon && 16 * t + li < K) ? widen(x[s][t]) : 0.0;
+                const double bv = (kon && NT * li + t < K) ? widen(x[s][t]) : 0.0;
                 acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[t], 0, 0, 0);
             }
         }
     }
 }
 
-// One block row per wavefront, entries [q0, q1) of big_brows wave-
-// interleaved.  Two slots are in flight: while slot u feeds the MFMAs, the
-// operands of slots u+1 and u+2 are being loaded (past the chunk's last slot
-// the loads repeat it; those products are never issued).
-template <typename T, typename V, int NT, bool DOT>
+// Block rows [q0, q1) of the matrix-core list, wave-interleaved (row q0 +
+// wave, then + kWaves, ...), one block row at a time per wavefront.  Within
+// a row two slots are in flight: while slot u feeds the MFMAs, the operands
+// of slots u+1 and u+2 are being loaded (past the chunk's last slot the loads
+// repeat it; those products are never issued).  Across rows, the next row's
+// descriptor is loaded when a row starts and its first metadata chunk before
+// the epilogue, so a row begins with its operand loads.
+template <typename T, typename V, int NT, bool VX, bool DOT>
 __device__ __forceinline__ void mfma_rows(const ApplyArgs<T, V>& g, int64_t q0, int64_t q1, int lane, int wave,
                                           double& d1, double& d2) {
     const DevPattern& p = g.p;
     const int li = lane & 15, lk = lane >> 4;
     const int K = g.K;
-    // the row header is wave-uniform: readfirstlane keeps the slot loop a
+    // the row state is wave-uniform: readfirstlane keeps the slot loop a
     // scalar loop (the compiler cannot see that the wave index is uniform)
-    for (int64_t q = q0 + __builtin_amdgcn_readfirstlane(wave); q < q1; q += kWaves) {
-        const int i = __builtin_amdgcn_readfirstlane(p.big_brows[q]);
-        const int r = __builtin_amdgcn_readfirstlane(p.rbs[i]);
-        const int kb = __builtin_amdgcn_readfirstlane(p.row_ptr[i]);
-        const int ke = __builtin_amdgcn_readfirstlane(p.row_ptr[i + 1]);
-        const int64_t vrow = uniform64(p.voff[kb]);
+    int64_t q = q0 + __builtin_amdgcn_readfirstlane(wave);
+    if (q >= q1) return;
+    // metadata of the first chunk of a row: one slot per lane (clamped; an
+    // empty row reads a neighbouring valid slot and never uses it)
+    auto chunk_meta = [&](int kb, int ke, int& mx, int& mw) {
+        const int nb = ke - kb < 64 ? ke - kb : 64;
+        const int idx = nb > 0 ? kb + (lane < nb ? lane : nb - 1) : (kb > 0 ? kb - 1 : 0);
+        const int2 mm = p.meta[idx];
+        mx = mm.x;
+        mw = mm.y;
+    };
+    int4 h0 = p.big_desc[2 * q], h1 = p.big_desc[2 * q + 1];
+    int mx = 0, mw = 0;
+    chunk_meta(__builtin_amdgcn_readfirstlane(h0.x), __builtin_amdgcn_readfirstlane(h0.y), mx, mw);
+    for (;;) {
+        const int kb = __builtin_amdgcn_readfirstlane(h0.x), ke = __builtin_amdgcn_readfirstlane(h0.y);
+        const int r = __builtin_amdgcn_readfirstlane(h0.z);
+        const int64_t row0 = __builtin_amdgcn_readfirstlane(h0.w), x0r = __builtin_amdgcn_readfirstlane(h1.x);
+        const int64_t vrow = int64_t((uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(h1.z))) << 32) |
+                                     uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(h1.y))));
+        // the next row's descriptor (this row's again at the end of the range)
+        const int64_t qn = q + kWaves;
+        const int64_t qc = qn < q1 ? qn : q;
+        const int4 n0 = p.big_desc[2 * qc], n1 = p.big_desc[2 * qc + 1];
         vbh_d4 acc[NT];
 #pragma unroll
         for (int t = 0; t < NT; ++t) acc[t] = vbh_d4{0.0, 0.0, 0.0, 0.0};
         for (int t0 = kb; t0 < ke; t0 += 64) {
             const int nb = ke - t0 < 64 ? ke - t0 : 64;
-            // the 64 slots' metadata, one per lane; read back with readlane
-            int mx = 0, mw = 0;
-            {
-                const int2 mm = p.meta[t0 + (lane < nb ? lane : nb - 1)];
-                mx = mm.x;
-                mw = mm.y;
-            }
-            V a0[4], a1[4];
+            if (t0 > kb) chunk_meta(t0, ke, mx, mw);  // rows of more than 64 slots
+            V a0[2][2], a1[2][2];
             T x0[4][NT], x1[4][NT];
             int c0 = 0, c1 = 0;
-            mfma_load<T, V, NT>(g, mx, mw, 0, vrow, r, li, lk, a0, x0, c0);
-            mfma_load<T, V, NT>(g, mx, mw, nb > 1 ? 1 : 0, vrow, r, li, lk, a1, x1, c1);
+            mfma_load<T, V, NT, VX>(g, mx, mw, 0, vrow, r, li, lk, a0, x0, c0);
+            mfma_load<T, V, NT, VX>(g, mx, mw, nb > 1 ? 1 : 0, vrow, r, li, lk, a1, x1, c1);
             for (int u = 0; u < nb; u += 2) {
                 mfma_slot<T, V, NT>(c0, r, li, lk, K, a0, x0, acc);
-                mfma_load<T, V, NT>(g, mx, mw, u + 2 < nb ? u + 2 : nb - 1, vrow, r, li, lk, a0, x0, c0);
+                mfma_load<T, V, NT, VX>(g, mx, mw, u + 2 < nb ? u + 2 : nb - 1, vrow, r, li, lk, a0, x0, c0);
                 if (u + 1 < nb) mfma_slot<T, V, NT>(c1, r, li, lk, K, a1, x1, acc);
-                mfma_load<T, V, NT>(g, mx, mw, u + 3 < nb ? u + 3 : nb - 1, vrow, r, li, lk, a1, x1, c1);
+                mfma_load<T, V, NT, VX>(g, mx, mw, u + 3 < nb ? u + 3 : nb - 1, vrow, r, li, lk, a1, x1, c1);
             }
         }
-        // register e of lane l holds row 4e + l/16 of column 16t + l%16.  All
-        // epilogue loads (clamped rows and columns) are issued before the
-        // first use, so the row costs one more latency, not one per element
-        const int64_t row0 = p.roff[i], x0r = p.coff[i];
+        // the next row's first metadata chunk, in flight during the epilogue
+        chunk_meta(__builtin_amdgcn_readfirstlane(n0.x), __builtin_amdgcn_readfirstlane(n0.y), mx, mw);
+        // register e of lane l holds row 4e + l/16; tile t holds column
+        // NT*(l%16) + t.  All epilogue loads (clamped rows and columns) are
+        // issued before the first use, so the row costs one more latency,
+        // not one per element
         const bool need_x = DOT || g.beta != 0.0, need_z = g.gamma != 0.0, need_w = DOT && g.W != nullptr;
-        double xo[NT][4], zo[NT][4], wo[NT][4];
+        const int c0 = NT * li < K - NT ? NT * li : K - NT;
+        T xo[4][NT], zo[4][NT], wo[4][NT];
 #pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int col = 16 * t + li;
-            const int cc = col < K ? col : K - 1;
+        for (int e = 0; e < 4; ++e) {
+            const int a = 4 * e + lk;
+            const int ac = a < r ? a : r - 1;
+            const T* xp = g.X + (x0r + ac) * g.ldx;
+            const T* zp = g.Z + (row0 + ac) * g.ldz;
+            const T* wp = g.W + (row0 + ac) * g.ldw;
+            if constexpr (VX) {
+                if (need_x) load_n<T, NT>(xp + c0, xo[e]);
+                if (need_z) load_n<T, NT>(zp + c0, zo[e]);
+                if (need_w) load_n<T, NT>(wp + c0, wo[e]);
+            } else {
 #pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int a = 4 * e + lk;
-                const int ac = a < r ? a : r - 1;
-                xo[t][e] = need_x ? widen(g.X[(x0r + ac) * g.ldx + cc]) : 0.0;
-                zo[t][e] = need_z ? widen(g.Z[(row0 + ac) * g.ldz + cc]) : 0.0;
-                wo[t][e] = need_w ? widen(g.W[(row0 + ac) * g.ldw + cc]) : 0.0;
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < NT; ++t) {
-            const int col = 16 * t + li;
-#pragma unroll
-            for (int e = 0; e < 4; ++e) {
-                const int a = 4 * e + lk;
-                const bool on = a < r && col < K;
-                double y = g.alpha * acc[t][e];
-                if (g.beta != 0.0) y += g.beta * xo[t][e];
-                if (need_z) y += g.gamma * zo[t][e];
-                const T ys = narrow_to<T>(y);
-                if (on) g.Y[(row0 + a) * g.ldy + col] = ys;
-                if constexpr (DOT) {
-                    const double yw = widen(ys);
-                    d1 += on ? (need_w ? wo[t][e] : yw) * yw : 0.0;
-                    d2 += on ? xo[t][e] * yw : 0.0;
+                for (int t = 0; t < NT; ++t) {
+                    const int col = NT * li + t;
+                    const int cc = col < K ? col : K - 1;
+                    if (need_x) xo[e][t] = xp[cc];
+                    if (need_z) zo[e][t] = zp[cc];
+                    if (need_w) wo[e][t] = wp[cc];
                 }
             }
         }
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int a = 4 * e + lk;
+            T ys[NT];
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const bool on = a < r && NT * li + t < K;
+                const double xv = need_x ? widen(xo[e][t]) : 0.0;
+                double y = g.alpha * acc[t][e];
+                if (g.beta != 0.0) y += g.beta * xv;
+                if (need_z) y += g.gamma * widen(zo[e][t]);
+                ys[t] = narrow_to<T>(y);
+                if constexpr (DOT) {
+                    const double yw = widen(ys[t]);
+                    d1 += on ? (need_w ? widen(wo[e][t]) : yw) * yw : 0.0;
+                    d2 += on ? xv * yw : 0.0;
+                }
+            }
+            T* yp = g.Y + (row0 + a) * g.ldy;
+            if constexpr (VX) {
+                if (a < r && NT * li < K) store_n<T, NT>(yp + NT * li, ys);
+            } else {
+#pragma unroll
+                for (int t = 0; t < NT; ++t)
+                    if (a < r && NT * li + t < K) yp[NT * li + t] = ys[t];
+            }
+        }
+        if (qn >= q1) break;
+        q = qn;
+        h0 = n0;
+        h1 = n1;
     }
 }
 
-template <typename T, typename V, int NT, bool DOT>
+template <typename T, typename V, int NT, bool VX, bool DOT>
 __global__ __launch_bounds__(kThreads) void k_apply_mfma(ApplyArgs<T, V> g) {
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
@@ -549,13 +626,13 @@ __global__ __launch_bounds__(kThreads) void k_apply_mfma(ApplyArgs<T, V> g) {
     const int64_t q0 = int64_t(chunk) * kMfmaRowsPerGroup;
     const int64_t q1 = q0 + kMfmaRowsPerGroup < g.p.n_big ? q0 + kMfmaRowsPerGroup : g.p.n_big;
     double d1 = 0.0, d2 = 0.0;
-    mfma_rows<T, V, NT, DOT>(g, q0, q1, lane, wave, d1, d2);
+    mfma_rows<T, V, NT, VX, DOT>(g, q0, q1, lane, wave, d1, d2);
     if constexpr (DOT) write_partials(1, lane, wave, d1, d2, g.partials + int64_t(chunk) * 2);
 }
 
 // The matrix-core class of every matrix of a batch in one launch: work-group
 // gg takes kMfmaRowsPerGroup entries of its matrix's big_brows.
-template <typename T, typename V, int NT>
+template <typename T, typename V, int NT, bool VX>
 __global__ __launch_bounds__(kThreads) void k_batched_mfma(const BatchItem<T, V>* __restrict__ items, int nitems,
                                                            int Gtot, int K, int ld, int step, double alpha,
                                                            double beta, double gamma0, double gamma, int mode,
@@ -570,7 +647,7 @@ __global__ __launch_bounds__(kThreads) void k_batched_mfma(const BatchItem<T, V>
     const int64_t q0 = int64_t(gg - it.g_base2) * kMfmaRowsPerGroup;
     const int64_t q1 = q0 + kMfmaRowsPerGroup < a.p.n_big ? q0 + kMfmaRowsPerGroup : a.p.n_big;
     double d1 = 0.0, d2 = 0.0;
-    if (it.row_list) mfma_rows<T, V, NT, true>(a, q0, q1, lane, wave, d1, d2);
+    if (it.row_list) mfma_rows<T, V, NT, VX, true>(a, q0, q1, lane, wave, d1, d2);
     write_partials(1, lane, wave, d1, d2, partials + int64_t(gg) * 2);
 }
 
@@ -666,8 +743,12 @@ void launch_batched_mfma(const BatchItem<T, V>* items, int nitems, int Gtot, int
     } else {
         auto go = [&](auto NT_) {
             constexpr int NT = decltype(NT_)::value;
-            hipLaunchKernelGGL((k_batched_mfma<T, V, NT>), dim3(Gtot), dim3(kThreads), 0, s, items, nitems, Gtot, K,
-                               ld, step, alpha, beta, gamma0, gamma, mode, partials);
+            if (K % NT == 0)
+                hipLaunchKernelGGL((k_batched_mfma<T, V, NT, true>), dim3(Gtot), dim3(kThreads), 0, s, items, nitems,
+                                   Gtot, K, ld, step, alpha, beta, gamma0, gamma, mode, partials);
+            else
+                hipLaunchKernelGGL((k_batched_mfma<T, V, NT, false>), dim3(Gtot), dim3(kThreads), 0, s, items, nitems,
+                                   Gtot, K, ld, step, alpha, beta, gamma0, gamma, mode, partials);
         };
         switch ((K + 15) / 16) {
             case 1: go(std::integral_constant<int, 1>{}); break;
@@ -689,8 +770,11 @@ void launch_mfma(const ApplyArgs<T, V>& a, hipStream_t s) {
         const dim3 grid(a.G), block(kThreads);
         auto go = [&](auto NT_) {
             constexpr int NT = decltype(NT_)::value;
-            if (dot) hipLaunchKernelGGL((k_apply_mfma<T, V, NT, true>), grid, block, 0, s, a);
-            else hipLaunchKernelGGL((k_apply_mfma<T, V, NT, false>), grid, block, 0, s, a);
+            const bool vx = a.K % NT == 0;
+            if (dot && vx) hipLaunchKernelGGL((k_apply_mfma<T, V, NT, true, true>), grid, block, 0, s, a);
+            else if (dot) hipLaunchKernelGGL((k_apply_mfma<T, V, NT, false, true>), grid, block, 0, s, a);
+            else if (vx) hipLaunchKernelGGL((k_apply_mfma<T, V, NT, true, false>), grid, block, 0, s, a);
+            else hipLaunchKernelGGL((k_apply_mfma<T, V, NT, false, false>), grid, block, 0, s, a);
         };
         switch (nt) {
             case 1: go(std::integral_constant<int, 1>{}); break;

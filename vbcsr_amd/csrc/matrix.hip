@@ -328,7 +328,14 @@ vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const
             if (!big.empty()) {
                 m->n_big = int(big.size());
                 m->n_small = int64_t(small.size());
-                m->big_brows = upload(big, s);
+                std::vector<int4> desc(2 * big.size());
+                for (size_t q = 0; q < big.size(); ++q) {
+                    const int i = big[q];
+                    const int64_t v = m->h_voff[row_ptr[i]];
+                    desc[2 * q] = make_int4(row_ptr[i], row_ptr[i + 1], rbs[i], roff[i]);
+                    desc[2 * q + 1] = make_int4(coff[i], int(uint32_t(uint64_t(v))), int(uint32_t(uint64_t(v) >> 32)), i);
+                }
+                m->big_desc = upload(desc, s);
                 m->small_rows = upload(small, s);
             }
         }
@@ -349,7 +356,7 @@ void destroy_matrix(vbh_matrix* m) {
     if (!m) return;
     for (void* p : {(void*)m->rbs, (void*)m->cbs, (void*)m->row_ptr, (void*)m->col_ind, (void*)m->roff,
                     (void*)m->coff, (void*)m->voff, (void*)m->meta, (void*)m->row_of, (void*)m->small_rows,
-                    (void*)m->big_brows, m->vals, m->vals32,
+                    (void*)m->big_desc, m->vals, m->vals32,
                     (void*)m->split, (void*)m->t_ptr,
                     (void*)m->t_row, (void*)m->t_slot, (void*)m->t_split})
         dfree(p);
