@@ -281,7 +281,9 @@ __device__ __forceinline__ void spmv_issue(const ApplyArgs<T>& g, int64_t vrow, 
         // branch: no per-lane wait); inside the run the index is clamped
         if (q == 0 || q * 64 < nvec) {
             const int e = q * 64 + lane;
-            stage[q] = src[e < nvec ? e : nvec - 1];
+            // non-temporal: the value stream is read once, and evict-first
+            // lines leave the L2 to the x entries, which are gathered again
+            stage[q] = __builtin_nontemporal_load(src + (e < nvec ? e : nvec - 1));
         }
     }
     // lane = p*RPP + b loads x[xo_p + b] (clamped into the block's width)

@@ -430,6 +430,16 @@ __device__ __forceinline__ void load_n(const T* src, T (&out)[N]) {
 #pragma unroll
     for (int e = 0; e < N; ++e) out[e] = v[e];
 }
+// The same as a non-temporal (evict-first) load, for data read once per
+// launch: the block values, whose lines should not displace gathered X rows.
+template <typename T, int N>
+__device__ __forceinline__ void load_n_stream(const T* src, T (&out)[N]) {
+    typedef T vec __attribute__((ext_vector_type(N)));
+    typedef T uvec __attribute__((ext_vector_type(N), aligned(sizeof(T))));
+    const vec v = __builtin_nontemporal_load(reinterpret_cast<const uvec*>(src));
+#pragma unroll
+    for (int e = 0; e < N; ++e) out[e] = v[e];
+}
 template <typename T, int N>
 __device__ __forceinline__ void store_n(T* dst, const T (&in)[N]) {
     typedef T vec __attribute__((ext_vector_type(N)));
@@ -458,7 +468,7 @@ __device__ __forceinline__ void mfma_load(const ApplyArgs<T, V>& g, int mx, int 
 #pragma unroll
     for (int m = 0; m < 2; ++m) {
         const int p0 = 8 * m + 2 * lk;
-        load_n<V, 2>(ab + (p0 < c ? p0 : c - 1), a[m]);
+        load_n_stream<V, 2>(ab + (p0 < c ? p0 : c - 1), a[m]);
     }
     const int K = g.K;
     const int c0 = NT * li < K - NT ? NT * li : K - NT;  // VX: lanes past K read the last NT columns
