@@ -378,9 +378,11 @@ def main():
             res["fp64"] = np.stack([KP.kpm_moments(Hm, v, Mb, mode="plain") for Hm, v in zip(mats, V0s)])
 
         bsteps = max(2, args.steps // 5)
-        tb = {}
+        tb, plans = {}, {}
         for prec in ("mixed", "fp32"):
             tb[prec] = timed(batched(prec), bsteps, 1, sync)
+            t_, p_, tuned_ = KP.last_plan()
+            plans[prec] = {"mfma_min_rows": t_, "slot_parts": p_, "autotuned": tuned_}
         t64 = timed(plain_fp64, max(1, bsteps // 2), 1, sync)
         errs = {}
         for prec in ("mixed", "fp32"):
@@ -397,7 +399,9 @@ def main():
                           "fp64_plain_moment_vecs_per_s": count * Mb * Kb / t64}
         for prec in ("mixed", "fp32"):
             out["batched"][prec] = {"moment_vecs_per_s": count * Mb * Kb / tb[prec],
-                                    "speedup_vs_fp64_plain": t64 / tb[prec], "max_dos_error": errs[prec]}
+                                    "speedup_vs_fp64_plain": t64 / tb[prec], "max_dos_error": errs[prec],
+                                    "plan": plans[prec]}
+            log(f"batched {prec}: plan {plans[prec]}")
             log(f"batched {prec}: {tb[prec] * 1e3:.2f} ms, speedup {t64 / tb[prec]:.2f}x vs fp64 plain "
                 f"{t64 * 1e3:.2f} ms, max DOS err {errs[prec]:.2e}")
 

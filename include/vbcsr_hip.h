@@ -151,6 +151,22 @@ VBH_API int vbh_kpm_moments_batched(int count, const vbh_matrix_t* ms, int K, co
  * vbh_kpm_moments_batched call on this process, and the number of kernel
  * launches (graph nodes) it covers.  Profiling aid; no reference counterpart. */
 VBH_API int vbh_kpm_last_step_time(double* ms, int64_t* launches);
+/* Tile autotuning of vbh_kpm_moments_batched (on by default).  When on, the
+ * first call on a batch (same matrix handles, K, precision, mode) times one
+ * recursion step for each candidate plan -- the matrix-core threshold (block
+ * rows of t..16 rows on the FP64 matrix cores, t in {4, 8, 17 = none}) and the
+ * lane launch's slot partitions -- and keeps the fastest for every later call
+ * on that batch.  Plans change the order of floating-point sums only.  No
+ * reference counterpart (the reference has no device tiles). */
+VBH_API int vbh_set_autotune(int on);
+/* Pin the plan of later vbh_kpm_moments_batched calls (threshold 1..17, slot
+ * partitions: a power of two in 1..64, clamped to the lane shape), bypassing
+ * the cache and the autotuner; threshold 0 releases it.  For reproducible
+ * runs and for tests of every plan.  No reference counterpart. */
+VBH_API int vbh_set_batch_plan(int threshold, int parts);
+/* Plan of the last vbh_kpm_moments_batched call: matrix-core threshold, slot
+ * partitions, and whether it came from timing (1) or the default rule (0). */
+VBH_API int vbh_kpm_last_plan(int* threshold, int* parts, int* tuned);
 VBH_API int vbh_cheb_apply(vbh_matrix_t m, int K, const void* X, int ldx, void* Y, int ldy,
                            int nterms, const double* coef, double center, double half_width,
                            int precision, void* stream);

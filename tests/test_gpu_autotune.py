@@ -1,0 +1,96 @@
+"""GPU checks of the batched-KPM tile plans: every plan the autotuner can
+pick (matrix-core threshold x slot partitions) gives the oracle's moments
+(FP64: 1e-11 relative; precision "mixed": 1e-11 against the oracle on the
+FP32-rounded matrix), the autotuner reports a timed plan from its candidate
+set, and a second call on the same batch reuses that plan bit for bit."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import relerr
+from test_gpu_kpm import mk, rounded32
+
+pytestmark = pytest.mark.gpu
+
+
+def batch(seed, count=4, K=8):
+    from vbcsr_amd import workloads as W
+
+    Hs = W.batched_hamiltonians(count=count, nb_lo=40, nb_hi=160, neighbours=8.0, seed=seed)
+    mats = [mk(A) for A in Hs]
+    V0s = []
+    for h, H in enumerate(mats):
+        v = H.create_multivector(K)
+        v.set_random(300 + h, normalize=False)
+        V0s.append(v)
+    return Hs, mats, V0s
+
+
+@pytest.fixture
+def kpm_api():
+    from vbcsr_amd import kpm as K
+
+    yield K
+    K.set_batch_plan(0)
+    K.set_autotune(True)
+
+
+@pytest.mark.parametrize("threshold", [4, 8, 17])
+@pytest.mark.parametrize("parts", [1, 2, 8])
+def test_every_plan_matches_oracle(kpm_api, threshold, parts):
+    K = kpm_api
+    Hs, mats, V0s = batch(11)
+    K.set_batch_plan(threshold, parts)
+    M = 30
+    for precision, graph in (("fp64", True), ("fp64", False), ("mixed", True)):
+        mu = K.kpm_moments_batched(mats, V0s, M, mode="two", precision=precision, graph=graph)
+        assert K.last_plan()[0] == threshold and not K.last_plan()[2]
+        for h in range(len(mats)):
+            A = Hs[h] if precision == "fp64" else rounded32(Hs[h])
+            ref = O.kpm_moments(O.Mat.of(A), np.asarray(V0s[h].to_numpy()), M, mode=1)
+            assert relerr(mu[h], ref) < 1e-11, (precision, graph, h)
+
+
+def test_autotuner_picks_a_candidate_and_keeps_it(kpm_api):
+    K = kpm_api
+    K.set_batch_plan(0)
+    K.set_autotune(True)
+    Hs, mats, V0s = batch(12)
+    M = 24
+    mu1 = K.kpm_moments_batched(mats, V0s, M, mode="two", precision="mixed")
+    t, p, tuned = K.last_plan()
+    assert tuned and t in (4, 8, 17) and p >= 1
+    mu2 = K.kpm_moments_batched(mats, V0s, M, mode="two", precision="mixed")
+    assert K.last_plan() == (t, p, True)
+    assert np.array_equal(mu1, mu2)
+    for h in range(len(mats)):
+        ref = O.kpm_moments(O.Mat.of(rounded32(Hs[h])), np.asarray(V0s[h].to_numpy()), M, mode=1)
+        assert relerr(mu1[h], ref) < 1e-11
+
+
+def test_autotune_off_uses_the_default_rule(kpm_api):
+    K = kpm_api
+    K.set_batch_plan(0)
+    K.set_autotune(False)
+    Hs, mats, V0s = batch(13)
+    mu = K.kpm_moments_batched(mats, V0s, 20, mode="two")
+    t, p, tuned = K.last_plan()
+    assert not tuned and t == 4
+    for h in range(len(mats)):
+        ref = O.kpm_moments(O.Mat.of(Hs[h]), np.asarray(V0s[h].to_numpy()), 20, mode=1)
+        assert relerr(mu[h], ref) < 1e-11
+
+
+def test_single_matrix_calls_keep_their_row_classes(kpm_api):
+    """A batched call under another plan leaves each matrix's own lists active."""
+    K = kpm_api
+    Hs, mats, V0s = batch(14, count=2)
+    K.set_batch_plan(17, 2)
+    K.kpm_moments_batched(mats, V0s, 10, mode="two")
+    K.set_batch_plan(0)
+    X = np.random.default_rng(0).standard_normal((Hs[0].n, 8))
+    Xv = mats[0].create_multivector(8)
+    Xv.from_numpy(X)
+    Y = np.asarray(mats[0].mult(Xv).to_numpy())
+    ref = O.spmm(O.Mat.of(Hs[0]), X)
+    assert relerr(Y, ref) < 1e-12

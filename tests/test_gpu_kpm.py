@@ -1,7 +1,7 @@
 """GPU KPM recursion and Chebyshev series against the oracle, the reference's
 golden moments (ex08 composition) and closed forms.  FP64 tolerance 1e-11
 relative on the moments; the FP32-mixed recursion is bounded by its DOS error
-(<= 1e-5 against FP64, the precision target of the north star)."""
+(<= 1e-5 against FP64, the precision bar of the FP32 modes)."""
 import numpy as np
 import pytest
 

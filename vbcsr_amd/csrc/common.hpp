@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -219,10 +220,21 @@ struct vbh_matrix {
     int64_t* voff = nullptr;
     int2* meta = nullptr;
     int* row_of = nullptr;
+    // active row-class lists (select_row_classes); one set per matrix-core
+    // threshold is kept in row_lists
     int* small_rows = nullptr;
     int4* big_desc = nullptr;
     int64_t n_small = 0;
     int n_big = 0;
+    int mfma_min_rows = 0;
+    struct RowLists {
+        int* small_rows = nullptr;
+        int4* big_desc = nullptr;
+        int64_t n_small = 0;
+        int n_big = 0;
+    };
+    std::map<int, RowLists> row_lists;
+    std::vector<int> h_roff, h_coff, h_bfs;  // host copies for building the lists
     // values
     void* vals = nullptr;
     void* vals32 = nullptr;  // FP32 copy, built on demand
@@ -256,6 +268,8 @@ T* upload(const std::vector<T>& h, hipStream_t s = nullptr) {
 void ensure_vals32(vbh_matrix* m, hipStream_t s);
 // Ensure the adjoint (column-major incoming) plan exists.
 void ensure_adjoint_plan(vbh_matrix* m, hipStream_t s);
+// Activate the row-class lists of matrix-core threshold t (built on first use).
+void select_row_classes(vbh_matrix* m, int t, hipStream_t s);
 // Build a matrix handle from host pattern arrays (no value upload).
 vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const int* row_ptr,
                          const int* col_ind, int dtype, hipStream_t s);

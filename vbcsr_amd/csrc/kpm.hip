@@ -17,6 +17,7 @@
 // matrices in one launch per step and captures all steps in one HIP graph.
 #include <algorithm>
 #include <cstring>
+#include <map>
 
 #include "apply.hpp"
 
@@ -326,62 +327,126 @@ static void dispatch_kpm(const vbh_matrix* m, int precision, F&& f) {
 }
 
 // ------------------------------------------------------------ batched driver
+// Work layout of a batch: lane groups per matrix (over the small rows when the
+// matrix has matrix-core rows, real types), matrix-core groups per matrix,
+// each launch's total rounded up to a multiple of 8 (trailing groups find no
+// rows); K == 1: row loop over each matrix's work-balanced split.
+struct BatchLayout {
+    std::vector<int> seg, seg2;
+    int Gtot1 = 0, Gtot2 = 0;
+};
+static BatchLayout batch_layout(int count, vbh_matrix* const* ms, const LaneShape& sh, bool mfma_ok) {
+    BatchLayout L;
+    L.seg.assign(count + 1, 0);
+    L.seg2.assign(count + 1, 0);
+    for (int h = 0; h < count; ++h) {
+        const int64_t rpg = sh.S ? lane_rows_per_group(sh.S, sh.P) : 1;
+        const bool split = mfma_ok && ms[h]->n_big > 0;
+        const int64_t lrows = split ? ms[h]->n_small : ms[h]->n_rows;
+        L.seg[h + 1] = L.seg[h] + (sh.S ? int((lrows + rpg - 1) / rpg) : ms[h]->G);
+        L.seg2[h + 1] = L.seg2[h] + (split ? (ms[h]->n_big + kMfmaRowsPerGroup - 1) / kMfmaRowsPerGroup : 0);
+    }
+    L.Gtot1 = (L.seg[count] + 7) & ~7;
+    L.Gtot2 = (L.seg2[count] + 7) & ~7;
+    return L;
+}
+
+// Tile plan of a batch: the matrix-core threshold (block rows of t..16 rows
+// on the matrix cores, t = 17: none) and the lane launch's slot partitions.
+// With autotuning on, the first call on a batch times one recursion step of
+// every candidate and keeps the fastest for later calls on the same batch.
+struct BatchPlan {
+    int t = kMfmaMinRows;
+    int P = 1;
+    bool tuned = false;
+};
+static std::map<std::vector<uintptr_t>, BatchPlan> g_batch_plans;
+static int g_autotune = 1;
+static BatchPlan g_last_plan;
+static BatchPlan g_forced_plan{0, 1, false};  // t > 0: used instead of the cached or tuned plan
+
 template <typename T, typename S, typename V>
 static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* const* V0s, const int* ldv, int M, int mode,
                         double center, double half, int use_graph, double* moments, hipStream_t s) {
     VBH_REQUIRE(mode == 1 || mode == 2, "batched KPM supports the fused modes 1 and 2");
     VBH_REQUIRE(K >= 1 && K <= 64, "batched KPM supports 1 <= K <= 64");
+    using Item = BatchItem<T, V>;
     const int64_t nsteps = step_count<T>(M, mode);
-    std::vector<BatchItem<T, V>> items(count);
-    std::vector<int> seg(count + 1, 0), seg2(count + 1, 0);
+    std::vector<Item> items(count);
     std::vector<size_t> off(count + 1, 0);
-    // K >= 2: lane family, G_h = ceil(rows / rows-per-group) -- over the small
-    // rows when the matrix has a matrix-core class (real types), whose block
-    // rows run as a second launch with G2_h = ceil(big rows / 16); K == 1: row
-    // loop over each matrix's work-balanced split.  Dots are summed over the
-    // columns (E = 2) everywhere.
-    LaneShape sh = lane_shape_of<T, V>(K);
-    bool mfma_ok = !Sc<T>::is_complex && sh.S > 0;
-    for (int h = 0; h < count; ++h) mfma_ok = mfma_ok && ms[h]->max_block <= 16;
-    if (sh.S) {  // slot partitions from the batch's total lane rows
-        int64_t lrows = 0;
-        for (int h = 0; h < count; ++h)
-            lrows += (mfma_ok && ms[h]->n_big > 0) ? ms[h]->n_small : ms[h]->n_rows;
-        sh.P = lane_parts(lrows, sh.S, 0, int(sizeof(T)));
-    }
     for (int h = 0; h < count; ++h) {
         VBH_REQUIRE(ms[h]->nbc == ms[h]->nbr, "batched KPM needs square local patterns without ghosts");
         off[h + 1] = off[h] + vec_bytes_of<T>(ms[h]->n_rows, K) * (mode == 1 ? 4 : 3);
-        const int64_t rpg = sh.S ? lane_rows_per_group(sh.S, sh.P) : 1;
-        const bool split = mfma_ok && ms[h]->n_big > 0;
-        const int64_t lrows = split ? ms[h]->n_small : ms[h]->n_rows;
-        seg[h + 1] = seg[h] + (sh.S ? int((lrows + rpg - 1) / rpg) : ms[h]->G);
-        seg2[h + 1] = seg2[h] + (split ? (ms[h]->n_big + kMfmaRowsPerGroup - 1) / kMfmaRowsPerGroup : 0);
     }
-    const int Gtot1 = (seg[count] + 7) & ~7;  // trailing groups find no rows
-    const int Gtot2 = (seg2[count] + 7) & ~7;
-    const int Gtot = Gtot1 + Gtot2;
+    // Dots are summed over the columns (E = 2) everywhere.
+    LaneShape sh = lane_shape_of<T, V>(K);
+    bool mfma_ok = !Sc<T>::is_complex && sh.S > 0;
+    for (int h = 0; h < count; ++h) mfma_ok = mfma_ok && ms[h]->max_block <= 16;
+    std::vector<int> t_before(count);
+    for (int h = 0; h < count; ++h) t_before[h] = ms[h]->mfma_min_rows;
+    auto select_all = [&](int t) {
+        for (int h = 0; h < count; ++h) select_row_classes(ms[h], t, s);
+    };
+    auto default_parts = [&]() {
+        int64_t lrows = 0;
+        for (int h = 0; h < count; ++h)
+            lrows += (mfma_ok && ms[h]->n_big > 0) ? ms[h]->n_small : ms[h]->n_rows;
+        return lane_parts(lrows, sh.S, 0, int(sizeof(T)));
+    };
+    std::vector<uintptr_t> pkey = {uintptr_t(count), uintptr_t(K), uintptr_t(sizeof(T)), uintptr_t(sizeof(V)),
+                                   uintptr_t(mode)};
+    for (int h = 0; h < count; ++h) {
+        pkey.push_back(uintptr_t(ms[h]));
+        pkey.push_back(uintptr_t(ms[h]->nnzb));
+    }
+    BatchPlan plan;
+    auto known = g_batch_plans.find(pkey);
+    const bool forced = g_forced_plan.t > 0 && sh.S > 0;
+    const bool tune = !forced && sh.S > 0 && g_autotune && nsteps >= 2 && known == g_batch_plans.end();
+    if (forced) {
+        plan = g_forced_plan;
+        plan.P = std::min(plan.P, 64 / sh.S);
+    } else if (known != g_batch_plans.end()) {
+        plan = known->second;
+    } else if (sh.S) {
+        select_all(kMfmaMinRows);
+        plan.P = default_parts();
+    }
+    select_all(plan.t);
+    sh.P = sh.S ? plan.P : 1;
+    BatchLayout lay = batch_layout(count, ms, sh, mfma_ok);
+    std::vector<int>& seg = lay.seg;
+    std::vector<int>& seg2 = lay.seg2;
+    int Gtot1 = lay.Gtot1, Gtot2 = lay.Gtot2;
+    int Gtot = Gtot1 + Gtot2;
     const int E = 2;
     char* vecs = (char*)scratch(std::max<size_t>(off[count], 64), 4);
-    double* partials = (double*)scratch(sizeof(double) * std::max<int64_t>(nsteps, 1) * Gtot * E, 5);
+    // plan-dependent fields of the item table: the active row lists and the
+    // groups of every matrix in the layout
+    auto fill_items = [&](const BatchLayout& L) {
+        for (int h = 0; h < count; ++h) {
+            vbh_matrix* m = ms[h];
+            Item& it = items[h];
+            it.p = m->pattern();
+            it.p.bs = 0;
+            it.split = m->split;
+            it.n_rows = m->n_rows;
+            it.row_list = (mfma_ok && m->n_big > 0) ? 1 : 0;
+            it.lane_rows = it.row_list ? m->n_small : m->n_rows;
+            it.G = L.seg[h + 1] - L.seg[h];
+            it.g_base = L.seg[h];
+            it.g_base2 = L.seg2[h];
+        }
+    };
     std::vector<double> mu0(count, 0.0);
     for (int h = 0; h < count; ++h) {
         vbh_matrix* m = ms[h];
         const int64_t N = m->n_rows;
         const size_t vb = vec_bytes_of<T>(N, K);
-        BatchItem<T, V>& it = items[h];
-        it.p = m->pattern();
-        it.p.bs = 0;
+        Item& it = items[h];
         it.vals = compute_values<V>(m, s);
         for (int q = 0; q < 3; ++q) it.buf[q] = (T*)(vecs + off[h] + q * vb);
         it.v0 = mode == 1 ? (const T*)(vecs + off[h] + 3 * vb) : nullptr;
-        it.split = m->split;
-        it.n_rows = N;
-        it.row_list = (mfma_ok && m->n_big > 0) ? 1 : 0;
-        it.lane_rows = it.row_list ? m->n_small : N;
-        it.G = seg[h + 1] - seg[h];
-        it.g_base = seg[h];
-        it.g_base2 = seg2[h];
         if (N * K > 0) {
             hipLaunchKernelGGL((k_copy_convert<S, T>), dim3(unsigned((N * K + 255) / 256)), dim3(256), 0, s, N, K,
                                (const S*)V0s[h], int64_t(ldv[h]), it.buf[0], int64_t(K));
@@ -396,8 +461,90 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
             for (int v = 0; v < K; ++v) mu0[h] += dd[2 * v];
         }
     }
+    const double a1 = 1.0 / half, b1 = -center / half;
+    const double a2 = 2.0 / half, b2 = -2.0 * center / half;
+    static StreamFork fork;
+    if (sh.S && mfma_ok) fork.init();
+    // One recursion step of the lane family (and the forked matrix-core
+    // launch) for the items at ditems in layout L.
+    auto lane_step = [&](const Item* ditems, const BatchLayout& L, const LaneShape& shp, int64_t step, double* pt,
+                         hipStream_t st) {
+        const double al = step == 0 ? a1 : a2, be = step == 0 ? b1 : b2;
+        if (L.Gtot2 > 0) {
+            // the matrix-core launch of a step is independent of its lane
+            // launch: forked onto the side stream and joined back, so the two
+            // run side by side
+            VBH_CHECK_HIP(hipEventRecord(fork.fork, st));
+            VBH_CHECK_HIP(hipStreamWaitEvent(fork.side, fork.fork, 0));
+            launch_batched_mfma<T, V>(ditems, count, L.Gtot2, K, K, int(step), al, be, 0.0, -1.0, mode,
+                                      pt + int64_t(L.Gtot1) * E, fork.side);
+        }
+        if (L.Gtot1 > 0)
+            launch_batched_lanes<T, V>(ditems, count, L.Gtot1, K, K, shp, int(step), al, be, 0.0, -1.0, mode, pt, st);
+        if (L.Gtot2 > 0) {
+            VBH_CHECK_HIP(hipEventRecord(fork.join, fork.side));
+            VBH_CHECK_HIP(hipStreamWaitEvent(st, fork.join, 0));
+        }
+    };
+    if (tune) {
+        // Candidates: matrix-core thresholds (4: the 4..16-row classes on the
+        // matrix cores, 8: 8..16 only, 17: none) times slot partitions around
+        // the default.  Each is timed on recursion step 1 (reads buffers 0, 1,
+        // writes buffer 2; buffer 1 is cleared first), which the real run
+        // overwrites; its partials go to a scratch of their own.
+        for (int h = 0; h < count; ++h) {
+            const size_t vb = vec_bytes_of<T>(ms[h]->n_rows, K);
+            VBH_CHECK_HIP(hipMemsetAsync(items[h].buf[1], 0, vb, s));
+        }
+        std::vector<int> ts = mfma_ok ? std::vector<int>{4, 8, 17} : std::vector<int>{plan.t};
+        const int pmax = 64 / sh.S < 16 ? 64 / sh.S : 16;
+        float best = 0.f;
+        hipEvent_t e0, e1;
+        VBH_CHECK_HIP(hipEventCreate(&e0));
+        VBH_CHECK_HIP(hipEventCreate(&e1));
+        for (int t : ts) {
+            select_all(t);
+            const int p0 = default_parts();
+            for (int P : {p0 / 2, p0, 2 * p0, 4 * p0}) {
+                if (P < 1 || P > pmax) continue;
+                LaneShape shc = sh;
+                shc.P = P;
+                BatchLayout Lc = batch_layout(count, ms, shc, mfma_ok);
+                fill_items(Lc);
+                Item* titems = (Item*)scratch(sizeof(Item) * count, 8);
+                double* tpart = (double*)scratch(sizeof(double) * std::max(Lc.Gtot1 + Lc.Gtot2, 8) * E, 9);
+                VBH_CHECK_HIP(hipMemcpyAsync(titems, items.data(), sizeof(Item) * count, hipMemcpyHostToDevice, s));
+                lane_step(titems, Lc, shc, 1, tpart, s);  // warm-up
+                VBH_CHECK_HIP(hipEventRecord(e0, s));
+                for (int rep = 0; rep < 3; ++rep) lane_step(titems, Lc, shc, 1, tpart, s);
+                VBH_CHECK_HIP(hipEventRecord(e1, s));
+                VBH_CHECK_HIP(hipEventSynchronize(e1));
+                float ms_ = 0.f;
+                VBH_CHECK_HIP(hipEventElapsedTime(&ms_, e0, e1));
+                if (!plan.tuned || ms_ < best) {
+                    best = ms_;
+                    plan.t = t;
+                    plan.P = P;
+                    plan.tuned = true;
+                }
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        g_batch_plans[pkey] = plan;
+        select_all(plan.t);
+        sh.P = plan.P;
+        lay = batch_layout(count, ms, sh, mfma_ok);
+        Gtot1 = lay.Gtot1;
+        Gtot2 = lay.Gtot2;
+        Gtot = Gtot1 + Gtot2;
+    } else if (!forced && known == g_batch_plans.end()) {
+        g_batch_plans[pkey] = plan;
+    }
+    g_last_plan = plan;
+    fill_items(lay);
+    double* partials = (double*)scratch(sizeof(double) * std::max<int64_t>(nsteps, 1) * std::max(Gtot, 8) * E, 5);
     if (nsteps > 0 && Gtot > 0) {
-        using Item = BatchItem<T, V>;
         Item* ditems = (Item*)scratch(sizeof(Item) * count + 2 * sizeof(int) * (count + 1), 6);
         int* dseg = (int*)(ditems + count);
         int* dseg2 = dseg + (count + 1);
@@ -405,35 +552,16 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
         VBH_CHECK_HIP(hipMemcpyAsync(dseg, seg.data(), sizeof(int) * (count + 1), hipMemcpyHostToDevice, s));
         VBH_CHECK_HIP(hipMemcpyAsync(dseg2, seg2.data(), sizeof(int) * (count + 1), hipMemcpyHostToDevice, s));
         VBH_CHECK_HIP(hipStreamSynchronize(s));
-        const double a1 = 1.0 / half, b1 = -center / half;
-        const double a2 = 2.0 / half, b2 = -2.0 * center / half;
         int max_block = 1;
         for (int h = 0; h < count; ++h) max_block = std::max(max_block, ms[h]->max_block);
         const int rb = row_config(K < 2 ? 2 : K, max_block, 0).rb;
-        static StreamFork fork;
-        if (sh.S && Gtot2 > 0) fork.init();
         auto launch_all = [&](hipStream_t st) {
             for (int64_t step = 0; step < nsteps; ++step) {
                 const Item* it = ditems;
                 const double al = step == 0 ? a1 : a2, be = step == 0 ? b1 : b2;
                 double* pt = partials + step * Gtot * E;
                 if (sh.S) {
-                    if (Gtot2 > 0) {
-                        // the matrix-core launch of a step is independent of
-                        // its lane launch: forked onto the side stream and
-                        // joined back, so the two run side by side
-                        VBH_CHECK_HIP(hipEventRecord(fork.fork, st));
-                        VBH_CHECK_HIP(hipStreamWaitEvent(fork.side, fork.fork, 0));
-                        launch_batched_mfma<T, V>(it, count, Gtot2, K, K, int(step), al, be, 0.0, -1.0, mode,
-                                                  pt + int64_t(Gtot1) * E, fork.side);
-                    }
-                    if (Gtot1 > 0)
-                        launch_batched_lanes<T, V>(it, count, Gtot1, K, K, sh, int(step), al, be, 0.0, -1.0, mode, pt,
-                                                   st);
-                    if (Gtot2 > 0) {
-                        VBH_CHECK_HIP(hipEventRecord(fork.join, fork.side));
-                        VBH_CHECK_HIP(hipStreamWaitEvent(st, fork.join, 0));
-                    }
+                    lane_step(ditems, lay, sh, step, pt, st);
                     continue;
                 }
                 if constexpr (!is_mixed<T, V>()) {  // K == 1, single type: row loop
@@ -514,6 +642,9 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
             if (M >= 1) moments[int64_t(h) * M] = mu0[h];
         }
     }
+    // the matrices keep their own row classes for later single-matrix calls
+    // (the captured graph holds the plan's lists, which stay allocated)
+    for (int h = 0; h < count; ++h) select_row_classes(ms[h], t_before[h], s);
 }
 
 // ------------------------------------------------------------ Chebyshev apply
@@ -626,6 +757,30 @@ VBH_API int vbh_kpm_moments_batched(int count, const vbh_matrix_t* ms, int K, co
             using V = std::remove_pointer_t<decltype(vty)>;
             kpm_batched<T, S, V>(count, ms, K, V0s, ldv, M, mode, center, half_width, graph, moments, (hipStream_t)stream);
         });
+    });
+}
+
+VBH_API int vbh_set_autotune(int on) {
+    return guard([&] { g_autotune = on ? 1 : 0; });
+}
+
+VBH_API int vbh_set_batch_plan(int threshold, int parts) {
+    return guard([&] {
+        VBH_REQUIRE(threshold == 0 || (threshold >= 1 && threshold <= 17 && parts >= 1 && parts <= 64 &&
+                                       (parts & (parts - 1)) == 0),
+                    "threshold must be 0 (release) or 1..17, with parts a power of two in 1..64");
+        g_forced_plan.t = threshold;
+        g_forced_plan.P = parts;
+        g_forced_plan.tuned = false;
+    });
+}
+
+VBH_API int vbh_kpm_last_plan(int* threshold, int* parts, int* tuned) {
+    return guard([&] {
+        VBH_REQUIRE(threshold && parts && tuned, "null argument");
+        *threshold = g_last_plan.t;
+        *parts = g_last_plan.P;
+        *tuned = g_last_plan.tuned ? 1 : 0;
     });
 }
 

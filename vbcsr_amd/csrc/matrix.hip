@@ -42,7 +42,7 @@ void dfree(void* p) {
 // Library scratch: a few grow-only device buffers reused across calls on the
 // current device.  Calls that share a slot must not overlap on one device.
 static std::mutex g_scratch_mu;
-static std::vector<std::pair<void*, size_t>> g_scratch(8, {nullptr, 0});
+static std::vector<std::pair<void*, size_t>> g_scratch(12, {nullptr, 0});
 void* scratch(size_t bytes, int slot) {
     std::lock_guard<std::mutex> lk(g_scratch_mu);
     auto& s = g_scratch.at(slot);
@@ -211,6 +211,54 @@ static std::vector<int> bfs_order(int nbr, const int* row_ptr, const int* col_in
     return order;
 }
 
+// Row-class lists for matrix-core threshold t (built once per threshold and
+// kept): block rows of t..16 rows of a variable-block matrix run on the
+// matrix cores (uniform 4 x 4 matrices keep the quad-shared lane path), the
+// scalar rows of all others on the lane family.  Both lists follow a
+// breadth-first order of the block graph: rows that run at the same time on
+// one XCD are then graph neighbours, so the X rows they gather overlap and
+// stay in that XCD's L2.  Each row is computed exactly as in index order; only
+// the order of the rows changes.
+void select_row_classes(vbh_matrix* m, int t, hipStream_t s) {
+    auto found = m->row_lists.find(t);
+    if (found == m->row_lists.end()) {
+        const int nbr = m->nbr;
+        const std::vector<int>& rbs = m->h_rbs;
+        auto is_big = [&](int i) { return m->uniform_bs != 4 && rbs[i] >= t && rbs[i] <= 16; };
+        bool any_big = false;
+        for (int i = 0; i < nbr && !any_big; ++i) any_big = is_big(i);
+        vbh_matrix::RowLists L;
+        if (any_big) {
+            if (m->h_bfs.empty()) m->h_bfs = bfs_order(nbr, m->h_row_ptr.data(), m->h_col_ind.data());
+            std::vector<int> small, big;
+            for (int i : m->h_bfs) {
+                if (is_big(i)) big.push_back(i);
+                else
+                    for (int r = m->h_roff[i]; r < m->h_roff[i + 1]; ++r) small.push_back(r);
+            }
+            std::vector<int4> desc(2 * big.size());
+            for (size_t q = 0; q < big.size(); ++q) {
+                const int i = big[q];
+                const int kb = m->h_row_ptr[i];
+                const uint64_t v = uint64_t(m->h_voff[kb]);
+                desc[2 * q] = make_int4(kb, m->h_row_ptr[i + 1], rbs[i], m->h_roff[i]);
+                desc[2 * q + 1] = make_int4(m->h_coff[i], int(uint32_t(v)), int(uint32_t(v >> 32)), i);
+            }
+            L.n_big = int(big.size());
+            L.n_small = int64_t(small.size());
+            L.big_desc = upload(desc, s);
+            L.small_rows = upload(small, s);
+        }
+        found = m->row_lists.emplace(t, L).first;
+    }
+    const vbh_matrix::RowLists& L = found->second;
+    m->mfma_min_rows = t;
+    m->small_rows = L.small_rows;
+    m->big_desc = L.big_desc;
+    m->n_small = L.n_small;
+    m->n_big = L.n_big;
+}
+
 vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const int* row_ptr, const int* col_ind,
                          int dtype, hipStream_t s) {
     VBH_REQUIRE(nbr >= 0 && nbc >= 0, "negative block counts");
@@ -304,41 +352,10 @@ vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const
             for (int i = 0; i < nbr; ++i)
                 for (int t = roff[i]; t < roff[i + 1]; ++t) row_of[t] = i;
             m->row_of = upload(row_of, s);
-            // matrix-core class: block rows of kMfmaMinRows..16 rows of a
-            // variable-block matrix (uniform 4 x 4 matrices keep the
-            // quad-shared lane path)
-            auto is_big = [&](int i) {
-                return m->uniform_bs != 4 && rbs[i] >= kMfmaMinRows && rbs[i] <= 16;
-            };
-            bool any_big = false;
-            for (int i = 0; i < nbr && !any_big; ++i) any_big = is_big(i);
-            std::vector<int> small, big;
-            if (any_big) {
-                // Both row lists follow a breadth-first order of the block
-                // graph: rows that run at the same time on one XCD are then
-                // graph neighbours, so the X rows they gather overlap and stay
-                // in that XCD's L2.  Each row is computed exactly as in index
-                // order; only the order of the rows changes.
-                for (int i : bfs_order(nbr, row_ptr, col_ind)) {
-                    if (is_big(i)) big.push_back(i);
-                    else
-                        for (int t = roff[i]; t < roff[i + 1]; ++t) small.push_back(t);
-                }
-            }
-            if (!big.empty()) {
-                m->n_big = int(big.size());
-                m->n_small = int64_t(small.size());
-                std::vector<int4> desc(2 * big.size());
-                for (size_t q = 0; q < big.size(); ++q) {
-                    const int i = big[q];
-                    const int64_t v = m->h_voff[row_ptr[i]];
-                    desc[2 * q] = make_int4(row_ptr[i], row_ptr[i + 1], rbs[i], roff[i]);
-                    desc[2 * q + 1] = make_int4(coff[i], int(uint32_t(uint64_t(v))), int(uint32_t(uint64_t(v) >> 32)), i);
-                }
-                m->big_desc = upload(desc, s);
-                m->small_rows = upload(small, s);
-            }
         }
+        m->h_roff = roff;
+        m->h_coff = coff;
+        select_row_classes(m, kMfmaMinRows, s);
         m->split = upload(split, s);
         // kVecPad trailing elements: the streaming kernels read whole 16-byte
         // words, which may reach up to one word past the last value
@@ -355,11 +372,14 @@ vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const
 void destroy_matrix(vbh_matrix* m) {
     if (!m) return;
     for (void* p : {(void*)m->rbs, (void*)m->cbs, (void*)m->row_ptr, (void*)m->col_ind, (void*)m->roff,
-                    (void*)m->coff, (void*)m->voff, (void*)m->meta, (void*)m->row_of, (void*)m->small_rows,
-                    (void*)m->big_desc, m->vals, m->vals32,
+                    (void*)m->coff, (void*)m->voff, (void*)m->meta, (void*)m->row_of, m->vals, m->vals32,
                     (void*)m->split, (void*)m->t_ptr,
                     (void*)m->t_row, (void*)m->t_slot, (void*)m->t_split})
         dfree(p);
+    for (auto& kv : m->row_lists) {
+        dfree(kv.second.small_rows);
+        dfree(kv.second.big_desc);
+    }
     delete m;
 }
 

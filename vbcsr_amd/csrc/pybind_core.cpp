@@ -964,6 +964,13 @@ PYBIND11_MODULE(_core, m) {
         check(vbh_kpm_last_step_time(&ms, &n), "kpm_last_step_time");
         return py::make_tuple(ms, n);
     });
+    m.def("set_autotune", [](bool on) { check(vbh_set_autotune(on ? 1 : 0), "set_autotune"); });
+    m.def("set_batch_plan", [](int t, int p) { check(vbh_set_batch_plan(t, p), "set_batch_plan"); });
+    m.def("kpm_last_plan", []() {
+        int t = 0, p = 0, tuned = 0;
+        check(vbh_kpm_last_plan(&t, &p, &tuned), "kpm_last_plan");
+        return py::make_tuple(t, p, bool(tuned));
+    });
     m.def("synchronize", []() { check(vbh_stream_sync(nullptr), "synchronize"); });
 
     py::enum_<AssemblyMode>(m, "AssemblyMode")

@@ -51,6 +51,24 @@ def kpm_moments_batched(Hs: Sequence[VBCSR], V0s: Sequence[DistMultiVector], num
                                                 float(half_width), _PREC[precision], bool(graph)))
 
 
+def set_autotune(on: bool) -> None:
+    """Tile autotuning of kpm_moments_batched (on by default): the first call
+    on a batch times one recursion step per candidate plan (matrix-core
+    threshold x slot partitions) and keeps the fastest for that batch."""
+    _core.set_autotune(bool(on))
+
+
+def set_batch_plan(threshold: int = 0, parts: int = 1) -> None:
+    """Pin the tile plan of later batched calls (matrix-core threshold 1..17,
+    17 = no matrix-core rows; slot partitions); threshold 0 releases it."""
+    _core.set_batch_plan(int(threshold), int(parts))
+
+
+def last_plan() -> Tuple[int, int, bool]:
+    """(matrix-core threshold, slot partitions, tuned) of the last batched call."""
+    return tuple(_core.kpm_last_plan())
+
+
 def chebyshev_nodes(num_points: int) -> np.ndarray:
     """Ascending Gauss-Chebyshev nodes x_k = -cos(pi (k + 1/2)/K); never +-1."""
     return -np.cos(np.pi * (np.arange(num_points) + 0.5) / num_points)
