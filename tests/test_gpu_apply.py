@@ -162,3 +162,21 @@ def test_graphene_bsr_fp64_fp32():
     X = np.random.default_rng(2).choice([-1.0, 1.0], size=(A.n, 10))
     assert relerr(H.mult(X).to_numpy(), O.spmm(m, X)) < FP64_TOL
     assert relerr(H.mult(X[:, 0]).to_numpy(), O.spmv(m, X[:, 0])) < FP64_TOL
+
+
+@pytest.mark.parametrize("bs_hi,cplx", [(8, False), (4, False), (4, True)])
+def test_large_spmv_vs_oracle(bs_hi, cplx):
+    """SpMV on matrices whose x exceeds one XCD's L2 (over 2 MB in FP64),
+    4-8 and 2-4 row blocks, real and complex, against the oracle."""
+    from vbcsr_amd import workloads as W
+
+    A = W.random_vbcsr(60000 if bs_hi == 8 else 90000, bs_lo=bs_hi // 2, bs_hi=bs_hi, per_row=6, seed=5)
+    assert A.n * 8 > (2 << 20)
+    rng = np.random.default_rng(9)
+    if cplx:
+        A.vals = A.vals + 1j * rng.uniform(-1, 1, A.vals.shape)
+    H = vb().from_block_csr(A.bs, A.row_ptr, A.col_ind, A.vals)
+    x = rng.uniform(-1, 1, A.n) + (1j * rng.uniform(-1, 1, A.n) if cplx else 0)
+    y = H.mult(x).to_numpy()
+    yr = O.spmm(O.Mat.of(A), x[:, None])[:, 0]
+    assert relerr(y, yr) < FP64_TOL
