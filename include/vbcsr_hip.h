@@ -248,6 +248,23 @@ VBH_API int vbh_matrix_axpby(vbh_matrix_t self, const double* alpha, vbh_matrix_
  * pairs, numeric pairs, C blocks before filter, after. */
 VBH_API int vbh_spgemm(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric, vbh_matrix_t* out,
                        int64_t* stats, void* stream);
+/* Upper block triangle of the thresholded SpGEMM (block columns >= the
+ * row; BlockSpMat::spmm_hermitian_upper, block_csr.hpp:1605-1616, symbolic
+ * upper_only common.hpp:502-513,625): the same rules as vbh_spgemm restricted
+ * to j >= i.  Square results without ghost columns. */
+VBH_API int vbh_spgemm_upper(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric, vbh_matrix_t* out,
+                             int64_t* stats, void* stream);
+/* Hermitian completion of an upper-triangle matrix
+ * (BlockSpMat::complete_hermitian, block_csr.hpp:1913-1920; mirror
+ * transpose.hpp:139-200; diagonal average block_csr.hpp:1948-1968): blocks on
+ * and above the diagonal kept, C_ji = U_ij^H for every strict-upper block,
+ * diagonal blocks D <- (D + D^H)/2; blocks below the diagonal are ignored. */
+VBH_API int vbh_matrix_complete_hermitian(vbh_matrix_t U, vbh_matrix_t* out, void* stream);
+/* C = A B for a product known to be Hermitian: vbh_spgemm_upper then
+ * vbh_matrix_complete_hermitian (BlockSpMat::spmm_hermitian,
+ * block_csr.hpp:1597-1601).  The result is exactly Hermitian. */
+VBH_API int vbh_spgemm_hermitian(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric, vbh_matrix_t* out,
+                                 int64_t* stats, void* stream);
 /* FP64 trace and Frobenius norm of (P^2 - P) for idempotency checks:
  * out[0] = tr P, out[1] = ||P2 - P||_F with P2 = P*P given. */
 VBH_API int vbh_matrix_trace(vbh_matrix_t m, double* out, void* stream);

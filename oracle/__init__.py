@@ -127,6 +127,25 @@ class Mat:
     def pat(self):
         return (self.nbr, self.nbc, _p(self.rbs), _p(self.cbs), _p(self.row_ptr), _p(self.col_ind))
 
+    @property
+    def voff(self):
+        """Value offset of every slot (slot order, row-major blocks)."""
+        rows = np.repeat(np.arange(self.nbr), np.diff(self.row_ptr))
+        sz = self.rbs[rows].astype(np.int64) * self.cbs[self.col_ind].astype(np.int64)
+        return np.concatenate([[0], np.cumsum(sz)])
+
+    def dense(self):
+        """Dense copy (small matrices only)."""
+        ro = np.concatenate([[0], np.cumsum(self.rbs)])
+        co = np.concatenate([[0], np.cumsum(self.cbs)])
+        D = np.zeros((ro[-1], co[-1]), self.vals.dtype)
+        vo = self.voff
+        for i in range(self.nbr):
+            for k in range(self.row_ptr[i], self.row_ptr[i + 1]):
+                j = self.col_ind[k]
+                D[ro[i]:ro[i + 1], co[j]:co[j + 1]] = self.vals[vo[k]:vo[k + 1]].reshape(self.rbs[i], self.cbs[j])
+        return D
+
 
 # ---------------------------------------------------------------- restatement
 def spmv(m: Mat, x: np.ndarray) -> np.ndarray:
@@ -170,12 +189,45 @@ def _take_result(h, s: str, nbr: int, dtype):
     return rp, ci, v
 
 
-def spgemm(A: Mat, B: Mat, thr: float) -> Mat:
+def spgemm(A: Mat, B: Mat, thr: float, upper: bool = False) -> Mat:
+    """Thresholded SpGEMM; upper: only block columns >= the row
+    (spmm_hermitian_upper, common.hpp:502-513,625)."""
     h = getattr(lib(), f"orc_spgemm_{A.s}")(
         A.nbr, A.nbc, B.nbc, _p(A.rbs), _p(A.cbs), _p(B.cbs), _p(A.row_ptr), _p(A.col_ind),
-        _p(A.vals), _p(B.row_ptr), _p(B.col_ind), _p(B.vals), ctypes.c_double(thr))
+        _p(A.vals), _p(B.row_ptr), _p(B.col_ind), _p(B.vals), ctypes.c_double(thr), ctypes.c_int(int(upper)))
     rp, ci, v = _take_result(h, A.s, A.nbr, A.vals.dtype)
     return Mat(A.rbs, rp, ci, v, cbs=B.cbs)
+
+
+def complete_hermitian(U: Mat) -> Mat:
+    """U + strict_lower(U^H) for an upper-triangle U (blocks below the
+    diagonal ignored); each diagonal block D becomes (D + D^H) * 0.5
+    (transpose.hpp:139-200 mirror; block_csr.hpp:1948-1968).  Python loops:
+    small matrices only."""
+    n = U.nbr
+    voff = U.voff
+    rows = [[] for _ in range(n)]  # (col, block values)
+    for i in range(n):
+        for k in range(U.row_ptr[i], U.row_ptr[i + 1]):
+            j = int(U.col_ind[k])
+            if j < i:
+                continue
+            r, c = int(U.rbs[i]), int(U.cbs[j])
+            blk = U.vals[voff[k]:voff[k + 1]].reshape(r, c)
+            if j == i:
+                rows[i].append((j, (blk + blk.conj().T) * 0.5))
+            else:
+                rows[i].append((j, blk.copy()))
+                rows[j].append((i, blk.conj().T.copy()))
+    rp = np.zeros(n + 1, np.int32)
+    ci, vals = [], []
+    for i in range(n):
+        for j, b in sorted(rows[i], key=lambda t: t[0]):
+            ci.append(j)
+            vals.append(b.ravel())
+        rp[i + 1] = len(ci)
+    v = np.concatenate(vals) if vals else np.zeros(0, U.vals.dtype)
+    return Mat(U.rbs, rp, np.asarray(ci, np.int32), v.astype(U.vals.dtype), cbs=U.cbs)
 
 
 def filter_blocks(m: Mat, thr: float) -> Mat:

@@ -160,9 +160,11 @@ struct Result {
 //            :371-372, skip :385-387, hashed destination :505-527).
 //  filter:   only when thr > 0, drop blocks with |C_ij|_F < thr, keep >= thr
 //            -- block_csr.hpp:1399-1528 (rule :1411), called at vbcsr.hpp:95.
+// upper: keep only block columns j >= i (the upper block triangle of a
+// square result, spmm_hermitian_upper -- common.hpp:502-513, skip :625).
 // A: nbr x nbk block rows/cols, B: nbk x nbc.
 template <typename T>
-Result<T>* spgemm(const Pattern& A, const T* av, const Pattern& B, const T* bv, double thr) {
+Result<T>* spgemm(const Pattern& A, const T* av, const Pattern& B, const T* bv, double thr, bool upper = false) {
     std::vector<double> an(A.row_ptr[A.nbr]), bn(B.row_ptr[B.nbr]);
     block_norms(A, av, an.data());
     block_norms(B, bv, bn.data());
@@ -175,6 +177,7 @@ Result<T>* spgemm(const Pattern& A, const T* av, const Pattern& B, const T* bv, 
             const int k = A.col_ind[ka];
             for (int kb = B.row_ptr[k]; kb < B.row_ptr[k + 1]; ++kb) {
                 const int j = B.col_ind[kb];
+                if (upper && j < i) continue;
                 auto it = est.find(j);
                 if (it == est.end()) est.emplace(j, an[ka] * bn[kb]);
                 else it->second += an[ka] * bn[kb];
@@ -444,10 +447,11 @@ ORC_API void orc_random_rademacher(int nblocks, const int* bs, const int* gids, 
     }                                                                                              \
     ORC_API void* orc_spgemm_##SFX(int nbr, int nbk, int nbc, const int* rbs, const int* kbs,      \
                                    const int* cbs, const int* arp, const int* aci, const void* av, \
-                                   const int* brp, const int* bci, const void* bv, double thr) {   \
+                                   const int* brp, const int* bci, const void* bv, double thr,     \
+                                   int upper) {                                                    \
         Pattern A(nbr, nbk, rbs, kbs, arp, aci);                                                   \
         Pattern B(nbk, nbc, kbs, cbs, brp, bci);                                                   \
-        return spgemm(A, (const T*)av, B, (const T*)bv, thr);                                      \
+        return spgemm(A, (const T*)av, B, (const T*)bv, thr, upper != 0);                          \
     }                                                                                              \
     ORC_API void* orc_filter_##SFX(PATARGS, const void* v, double thr) {                           \
         PAT;                                                                                       \

@@ -619,6 +619,40 @@ struct Mat {
         auto res = std::make_shared<Mat>(graph_for_pattern(out, *graph), out);
         return res;
     }
+    // spmm_hermitian (block_csr.hpp:1597-1601): the upper block triangle of
+    // op(A) B, then its Hermitian completion; spmm_hermitian_upper alone
+    // (:1605-1616) and complete_hermitian (:1913-1920).
+    std::shared_ptr<Mat> spmm_hermitian(Mat& B, double thr, bool transA, int numeric = 0) {
+        if (transA) return transpose()->spmm_hermitian(B, thr, false, numeric);
+        return herm_call(B, thr, numeric, vbh_spgemm_hermitian, "spmm_hermitian");
+    }
+    std::shared_ptr<Mat> spmm_hermitian_upper(Mat& B, double thr, bool transA, int numeric = 0) {
+        if (transA) return transpose()->spmm_hermitian_upper(B, thr, false, numeric);
+        return herm_call(B, thr, numeric, vbh_spgemm_upper, "spmm_hermitian_upper");
+    }
+    std::shared_ptr<Mat> complete_hermitian() {
+        if (graph->ghost_global_indices.size())
+            throw std::runtime_error("complete_hermitian: ghost columns need the distributed exchange");
+        vbh_matrix_t out = nullptr;
+        vbh_matrix_t a = dev();
+        {
+            py::gil_scoped_release nogil;
+            check(vbh_matrix_complete_hermitian(a, &out, nullptr), "complete_hermitian");
+        }
+        return std::make_shared<Mat>(graph_for_pattern(out, *graph), out);
+    }
+    template <typename F>
+    std::shared_ptr<Mat> herm_call(Mat& B, double thr, int numeric, F fn, const char* what) {
+        if (graph->ghost_global_indices.size() || B.graph->ghost_global_indices.size())
+            throw std::runtime_error(std::string(what) + ": ghost columns need the distributed exchange");
+        vbh_matrix_t out = nullptr;
+        vbh_matrix_t a = dev(), b = B.dev();
+        {
+            py::gil_scoped_release nogil;
+            check(fn(a, b, thr, numeric, &out, last_stats, nullptr), what);
+        }
+        return std::make_shared<Mat>(graph_for_pattern(out, *graph), out);
+    }
     static GraphP graph_for_pattern(vbh_matrix_t m, const Graph& rows_of) {
         vbh_matrix_info i{};
         check(vbh_matrix_info_get(m, &i), "info");
@@ -856,6 +890,11 @@ void bind_mat(py::module& m, const std::string& name) {
         .def("duplicate", &M::duplicate, py::arg("independent_graph") = true)
         .def("spmm", &M::spmm, py::arg("B"), py::arg("threshold"), py::arg("transA") = false,
              py::arg("transB") = false, py::arg("numeric") = 0)
+        .def("spmm_hermitian", &M::spmm_hermitian, py::arg("B"), py::arg("threshold"), py::arg("transA") = false,
+             py::arg("numeric") = 0)
+        .def("spmm_hermitian_upper", &M::spmm_hermitian_upper, py::arg("B"), py::arg("threshold"),
+             py::arg("transA") = false, py::arg("numeric") = 0)
+        .def("complete_hermitian", &M::complete_hermitian)
         .def("spmm_self",
              [](M& self, double thr, bool transA) { return self.spmm(self, thr, transA, false, 0); },
              py::arg("threshold"), py::arg("transA") = false)

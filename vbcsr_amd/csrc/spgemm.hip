@@ -62,6 +62,7 @@ struct SymArgs {
     int* out_count;         // per row count
     int64_t* pairs;         // per row multiplication paths
     double thr;
+    int upper;              // keep only block columns j >= the row i (upper triangle)
     int* g_keys;            // global tables (LDS == false)
     double* g_vals;
     const int64_t* g_off;   // per launch-row table offset
@@ -100,6 +101,7 @@ __global__ __launch_bounds__(256) void k_symbolic(SymArgs a) {
         npairs += b1 - b0;
         for (int kb = b0 + lane; kb < b1; kb += 64) {
             const int j = a.b_col_ind[kb];
+            if (a.upper && j < i) continue;  // common.hpp:625 (upper_only)
             const double v = na * a.b_norm[kb];
             int h = j & mask;
             while (true) {
@@ -337,10 +339,12 @@ static inline unsigned nblk4(int64_t n) { return unsigned((n + 3) / 4); }
 
 using namespace vbh;
 
-extern "C" VBH_API int vbh_spgemm(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric, vbh_matrix_t* out,
-                                  int64_t* stats, void* stream) {
-    return guard([&] {
+// C = A B, or its upper block triangle (block columns >= the row) when upper.
+static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric, int upper, vbh_matrix_t* out,
+                        int64_t* stats, void* stream) {
+    {
         VBH_REQUIRE(A && B && out, "null argument");
+        VBH_REQUIRE(!upper || (A->nbr == B->nbc), "the upper-triangle product needs a square result without ghosts");
         VBH_REQUIRE(A->dtype == B->dtype, "A and B must have the same dtype");
         VBH_REQUIRE(A->nbc == B->nbr, "spmm: A's local columns must be B's block rows");
         for (int k = 0; k < B->nbr; ++k)
@@ -402,6 +406,7 @@ extern "C" VBH_API int vbh_spgemm(vbh_matrix_t A, vbh_matrix_t B, double thr, in
         sa.out_count = d_count;
         sa.pairs = d_pairs;
         sa.thr = thr;
+        sa.upper = upper;
         // LDS classes: <=256, <=1024, <=4096 slots; wider rows use global tables
         const int classes[3] = {256, 1024, 4096};
         std::vector<int> rows_cls[4];
@@ -545,5 +550,130 @@ extern "C" VBH_API int vbh_spgemm(vbh_matrix_t A, vbh_matrix_t B, double thr, in
             stats[3] = C->nnzb;
         }
         *out = C;
+    }
+}
+
+// Hermitian completion of an upper-triangle matrix U: C keeps U's blocks on
+// and above the diagonal, gains C_ji = U_ij^H for every strict-upper block, and
+// each diagonal block D becomes (D + D^H)/2 (transpose.hpp:139-200,
+// block_csr.hpp:1913-1975).  src[k]: U's slot for C's slot k; mode[k]: 0 copy,
+// 1 conjugate transpose, 2 Hermitian average.  One thread per C value.
+template <typename T>
+__global__ void k_complete_hermitian(int64_t nvals, const int64_t* __restrict__ cvoff, int nnzb,
+                                     const int* __restrict__ crow, const int* __restrict__ ccol,
+                                     const int* __restrict__ rbs, const int64_t* __restrict__ uvoff,
+                                     const int* __restrict__ src, const int* __restrict__ mode,
+                                     const T* __restrict__ U, T* __restrict__ C) {
+    const int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e >= nvals) return;
+    // slot of value e: last k with cvoff[k] <= e
+    int lo = 0, hi = nnzb - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (cvoff[mid] <= e) lo = mid;
+        else hi = mid - 1;
+    }
+    const int k = lo;
+    const int c = rbs[ccol[k]];
+    const int64_t in = e - cvoff[k];
+    const int a = int(in / c), b = int(in - int64_t(a) * c);
+    const T* u = U + uvoff[src[k]];
+    const int m = mode[k];
+    if (m == 0) {
+        C[e] = u[int64_t(a) * c + b];
+    } else if (m == 1) {
+        // U block (j, i) is c x r with r = rbs[crow[k]]: element (b, a)
+        C[e] = Sc<T>::conj(u[int64_t(b) * rbs[crow[k]] + a]);
+    } else {
+        const T x = u[int64_t(a) * c + b], y = Sc<T>::conj(u[int64_t(b) * c + a]);
+        C[e] = rscale(0.5, x + y);
+    }
+}
+
+static void complete_hermitian_impl(vbh_matrix_t U, vbh_matrix_t* out, hipStream_t s) {
+    VBH_REQUIRE(U && out, "null argument");
+    VBH_REQUIRE(U->nbr == U->nbc, "Hermitian completion needs a square pattern without ghosts");
+    const int n = U->nbr;
+    for (int i = 0; i < n; ++i)
+        VBH_REQUIRE(U->h_rbs[i] == U->h_cbs[i], "Hermitian completion needs equal row and column block sizes");
+    const std::vector<int>& rp = U->h_row_ptr;
+    const std::vector<int>& ci = U->h_col_ind;
+    // rows of C: mirrored blocks (column j < i, from U's row j) then U's own
+    // blocks with j >= i; U blocks below the diagonal are ignored
+    std::vector<std::vector<std::pair<int, int>>> lower(n);  // (col, U slot)
+    for (int j = 0; j < n; ++j)
+        for (int k = rp[j]; k < rp[j + 1]; ++k)
+            if (ci[k] > j) lower[ci[k]].push_back({j, k});
+    std::vector<int> crp(n + 1, 0), cci, src, mode;
+    for (int i = 0; i < n; ++i) {
+        std::sort(lower[i].begin(), lower[i].end());
+        for (auto& q : lower[i]) {
+            cci.push_back(q.first);
+            src.push_back(q.second);
+            mode.push_back(1);
+        }
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            if (ci[k] < i) continue;
+            cci.push_back(ci[k]);
+            src.push_back(k);
+            mode.push_back(ci[k] == i ? 2 : 0);
+        }
+        crp[i + 1] = int(cci.size());
+    }
+    vbh_matrix* C = build_matrix(n, n, U->h_rbs.data(), U->h_cbs.data(), crp.data(), cci.data(), U->dtype, s);
+    try {
+        if (C->nnz > 0) {
+            std::vector<int> crow(C->nnzb);
+            for (int i = 0; i < n; ++i)
+                for (int k = crp[i]; k < crp[i + 1]; ++k) crow[k] = i;
+            int* d_src = upload(src, s);
+            int* d_mode = upload(mode, s);
+            int* d_crow = upload(crow, s);
+            const unsigned nb = unsigned((C->nnz + 255) / 256);
+            if (U->dtype == VBH_F64)
+                hipLaunchKernelGGL(k_complete_hermitian<double>, dim3(nb), dim3(256), 0, s, C->nnz, C->voff,
+                                   int(C->nnzb), d_crow, C->col_ind, C->rbs, U->voff, d_src, d_mode,
+                                   (const double*)U->vals, (double*)C->vals);
+            else
+                hipLaunchKernelGGL(k_complete_hermitian<c128>, dim3(nb), dim3(256), 0, s, C->nnz, C->voff,
+                                   int(C->nnzb), d_crow, C->col_ind, C->rbs, U->voff, d_src, d_mode,
+                                   (const c128*)U->vals, (c128*)C->vals);
+            VBH_CHECK_HIP(hipGetLastError());
+            VBH_CHECK_HIP(hipStreamSynchronize(s));
+            for (void* p : {(void*)d_src, (void*)d_mode, (void*)d_crow}) dfree(p);
+        }
+    } catch (...) {
+        destroy_matrix(C);
+        throw;
+    }
+    *out = C;
+}
+
+extern "C" VBH_API int vbh_spgemm(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric, vbh_matrix_t* out,
+                                  int64_t* stats, void* stream) {
+    return guard([&] { spgemm_impl(A, B, thr, numeric, 0, out, stats, stream); });
+}
+
+extern "C" VBH_API int vbh_spgemm_upper(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric, vbh_matrix_t* out,
+                                        int64_t* stats, void* stream) {
+    return guard([&] { spgemm_impl(A, B, thr, numeric, 1, out, stats, stream); });
+}
+
+extern "C" VBH_API int vbh_matrix_complete_hermitian(vbh_matrix_t U, vbh_matrix_t* out, void* stream) {
+    return guard([&] { complete_hermitian_impl(U, out, (hipStream_t)stream); });
+}
+
+extern "C" VBH_API int vbh_spgemm_hermitian(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
+                                            vbh_matrix_t* out, int64_t* stats, void* stream) {
+    return guard([&] {
+        vbh_matrix_t U = nullptr;
+        spgemm_impl(A, B, thr, numeric, 1, &U, stats, stream);
+        try {
+            complete_hermitian_impl(U, out, (hipStream_t)stream);
+        } catch (...) {
+            destroy_matrix(U);
+            throw;
+        }
+        destroy_matrix(U);
     });
 }

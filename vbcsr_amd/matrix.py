@@ -466,6 +466,34 @@ class VBCSR(LinearOperator):
         core_c = self._core.spmm(B._core, float(threshold), bool(transA), bool(transB), _NUMERIC[numeric])
         return self._wrap_core(core_c, self.dtype, self.comm, shape=(self.shape[0], B.shape[1]))
 
+    def spmm_hermitian(self, B: "VBCSR", threshold: float = 0.0, transA: bool = False,
+                       numeric: str = "fp64") -> "VBCSR":
+        """C = op(A) B for a product the caller knows is Hermitian
+        (vbcsr/matrix.py:770-796): only the upper block triangle is computed,
+        the strict lower triangle is its conjugate transpose and the diagonal
+        blocks are Hermitized, so the result is exactly Hermitian."""
+        if not isinstance(B, VBCSR):
+            raise TypeError("B must be a VBCSR matrix")
+        if self.dtype != B.dtype:
+            raise TypeError("A and B must have the same dtype")
+        core_c = self._core.spmm_hermitian(B._core, float(threshold), bool(transA), _NUMERIC[numeric])
+        return self._wrap_core(core_c, self.dtype, self.comm, shape=(self.shape[0], B.shape[1]))
+
+    def spmm_hermitian_upper(self, B: "VBCSR", threshold: float = 0.0, transA: bool = False,
+                             numeric: str = "fp64") -> "VBCSR":
+        """The upper block triangle (block columns >= the row) of op(A) B."""
+        if not isinstance(B, VBCSR):
+            raise TypeError("B must be a VBCSR matrix")
+        if self.dtype != B.dtype:
+            raise TypeError("A and B must have the same dtype")
+        core_c = self._core.spmm_hermitian_upper(B._core, float(threshold), bool(transA), _NUMERIC[numeric])
+        return self._wrap_core(core_c, self.dtype, self.comm, shape=(self.shape[0], B.shape[1]))
+
+    def complete_hermitian(self) -> "VBCSR":
+        """U + strict_lower(U^H) for an upper-triangle U, diagonal blocks
+        averaged with their conjugate transpose."""
+        return self._wrap_core(self._core.complete_hermitian(), self.dtype, self.comm, shape=self.shape)
+
     def spmm_self(self, threshold: float = 0.0, transA: bool = False) -> "VBCSR":
         return self._wrap_core(self._core.spmm_self(float(threshold), bool(transA)), self.dtype, self.comm,
                                shape=self.shape)
