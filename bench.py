@@ -202,7 +202,8 @@ def bench_mcweeny(args, sync, log):
     (threshold 1e-5), 5000 block rows of sizes 4..8, ~30 neighbours per row,
     P0 = 0.1 H + 0.5 I with H normalised by its Gershgorin bound (as written,
     without it, P0's spectrum leaves [-0.5, 1.5] and the iteration diverges);
-    numeric phase in FP64, FP32 and BF16 (matrix cores); idempotency
+    numeric phase in FP64 (full and Hermitian products), FP32 and BF16
+    (matrix cores); idempotency
     ||P^2 - P||_F and trace in FP64.  An iteration that passes the time budget
     ends the run for that precision (reported)."""
     from vbcsr_amd import VBCSR
@@ -213,7 +214,11 @@ def bench_mcweeny(args, sync, log):
     P0 = W.mcweeny_start(nb=nb, seed=3, normalize=True)
     res = {"workload": f"{nb} block rows, blocks 4-8, ~30 neighbours, Gershgorin-normalised H, threshold 1e-5, "
                        f"up to {iters} iterations"}
-    for numeric in ("fp64", "fp32", "bf16"):
+    # "fp64_hermitian": both products by spmm_hermitian (P, P^2 and P^3 are
+    # symmetric: only the upper block triangle is multiplied, then mirrored)
+    for leg in ("fp64", "fp64_hermitian", "fp32", "bf16"):
+        numeric = leg.split("_")[0]
+        herm = leg.endswith("hermitian")
         P = VBCSR.from_block_csr(P0.bs, P0.row_ptr, P0.col_ind, P0.vals)
         sync()
         t0 = time.perf_counter()
@@ -221,22 +226,26 @@ def bench_mcweeny(args, sync, log):
         per_iter = []
         for it in range(iters):
             ti = time.perf_counter()
-            P2 = P.spmm(P, 1e-5, numeric=numeric)
-            P3 = P2.spmm(P, 1e-5, numeric=numeric)
+            if herm:
+                P2 = P.spmm_hermitian(P, 1e-5, numeric=numeric)
+                P3 = P2.spmm_hermitian(P, 1e-5, numeric=numeric)
+            else:
+                P2 = P.spmm(P, 1e-5, numeric=numeric)
+                P3 = P2.spmm(P, 1e-5, numeric=numeric)
             P = P2 * 3.0 - P3 * 2.0
             sync()
             per_iter.append(time.perf_counter() - ti)
             done += 1
-            log(f"mcweeny {numeric} iteration {it}: {per_iter[-1] * 1e3:.1f} ms, {len(P.get_values())} values")
+            log(f"mcweeny {leg} iteration {it}: {per_iter[-1] * 1e3:.1f} ms, {len(P.get_values())} values")
             if time.perf_counter() - t0 > budget:
                 break
         secs = time.perf_counter() - t0
         P2 = P.spmm(P, 0.0)  # FP64 check: ||P^2 - P||_F over the union pattern, and tr P
         idem = float(np.linalg.norm((P2 - P).get_values()))
-        res[numeric] = {"iterations": done, "seconds": secs, "ms_per_iteration": [t * 1e3 for t in per_iter],
-                        "trace": float(P.trace()), "values_final": int(len(P.get_values())),
-                        "idempotency_frobenius": idem}
-        log(f"mcweeny {numeric}: {done} iterations in {secs:.2f} s, trace {res[numeric]['trace']:.6f}, "
+        res[leg] = {"iterations": done, "seconds": secs, "ms_per_iteration": [t * 1e3 for t in per_iter],
+                    "trace": float(P.trace()), "values_final": int(len(P.get_values())),
+                    "idempotency_frobenius": idem}
+        log(f"mcweeny {leg}: {done} iterations in {secs:.2f} s, trace {res[leg]['trace']:.6f}, "
             f"||P^2-P||_F {idem:.3e}")
     return res
 

@@ -355,7 +355,7 @@ __device__ inline void row_loop_spmv(const ApplyArgs<T>& g, int r0, int r1, int 
 #pragma unroll
                     for (int b = 0; b < RPP; ++b) {
                         const A t = Sc<A>::fma(widen(ab[b < cm ? b : cm]), widen(xs[b]), acc);
-                        acc = b < c ? t : acc;
+                        acc = pick(b < c, t, acc);
                     }
                 }
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

@@ -61,6 +61,17 @@ struct cplx {
 using c64 = cplx<float>;
 using c128 = cplx<double>;
 
+// a if c else b, member-wise for complex values: a select of a whole struct
+// compiles to a round trip through scratch memory.
+template <typename T>
+__host__ __device__ inline T pick(bool c, T a, T b) {
+    return c ? a : b;
+}
+template <typename R>
+__host__ __device__ inline cplx<R> pick(bool c, cplx<R> a, cplx<R> b) {
+    return {c ? a.re : b.re, c ? a.im : b.im};
+}
+
 template <typename T>
 struct Sc;  // scalar traits
 

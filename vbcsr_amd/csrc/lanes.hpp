@@ -177,7 +177,7 @@ __device__ __forceinline__ void quad_slots(const ApplyArgs<T, V>& g, int kb, int
                     t = Sc<A>::fma(widen(av[u][1]), widen(x1), t);
                     t = Sc<A>::fma(widen(av[u][2]), widen(x2), t);
                     t = Sc<A>::fma(widen(av[u][3]), widen(x3), t);
-                    acc[j] = on ? t : acc[j];
+                    acc[j] = pick(on, t, acc[j]);
                 }
             }
         }
@@ -285,7 +285,7 @@ __device__ __forceinline__ void lane_rows(const ApplyArgs<T, V>& g, int64_t sb, 
 #pragma unroll
                         for (int j = 0; j < KT; ++j) {
                             const A t = Sc<A>::fma(ab, widen(xv[u][j]), acc[j]);
-                            acc[j] = on ? t : acc[j];
+                            acc[j] = pick(on, t, acc[j]);
                         }
                     }
                 }
