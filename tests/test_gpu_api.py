@@ -138,3 +138,36 @@ def test_errors_are_python_exceptions():
     G = VBCSR.create_serial(2, [3, 2], [[0], [1]])
     with pytest.raises(RuntimeError):
         H.mult(G.create_vector())  # graph mismatch (dist_vector.hpp:54-70)
+
+
+@pytest.mark.parametrize("cplx", [False, True])
+def test_save_matrix_market(tmp_path, cplx):
+    """MatrixMarket export in the reference's format (block_csr.hpp:2058-2131):
+    the exact text of every line, and scipy's reader gives the dense matrix."""
+    import scipy.io
+
+    from vbcsr_amd import VBCSR
+    from vbcsr_amd import workloads as W
+
+    A = W.batched_hamiltonians(count=1, nb_lo=12, nb_hi=13, neighbours=4.0, seed=2, sizes=(1, 2, 3))[0]
+    vals = A.vals + (1j * np.random.default_rng(1).standard_normal(A.vals.shape) if cplx else 0)
+    H = VBCSR.from_block_csr(A.bs, A.row_ptr, A.col_ind, vals)
+    f = tmp_path / "m.mtx"
+    H.save_matrix_market(f)
+    lines = f.read_text().splitlines()
+    assert lines[0] == "%%MatrixMarket matrix coordinate " + ("complex" if cplx else "real") + " general"
+    off = A.offsets()
+    voff = A.value_offsets()
+    want = [f"{A.n} {A.n} {A.nnz}"]
+    for i in range(A.nb):
+        for k in range(A.row_ptr[i], A.row_ptr[i + 1]):
+            j = A.col_ind[k]
+            r, c = int(A.bs[i]), int(A.bs[j])
+            blk = vals[voff[k]:voff[k + 1]].reshape(r, c)
+            for a in range(r):
+                for b in range(c):
+                    v = blk[a, b]
+                    s = f"{v.real:.16e} {v.imag:.16e}" if cplx else f"{v:.16e}"
+                    want.append(f"{off[i] + a + 1} {off[j] + b + 1} {s}")
+    assert lines[1:] == want
+    assert np.array_equal(scipy.io.mmread(str(f)).toarray(), H.to_dense())

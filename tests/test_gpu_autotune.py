@@ -94,3 +94,31 @@ def test_single_matrix_calls_keep_their_row_classes(kpm_api):
     Y = np.asarray(mats[0].mult(Xv).to_numpy())
     ref = O.spmm(O.Mat.of(Hs[0]), X)
     assert relerr(Y, ref) < 1e-12
+
+
+@pytest.mark.parametrize("threshold", [4, 17])
+def test_batch_with_only_class_rows_or_only_lane_rows(kpm_api, threshold):
+    """Edge layouts: one matrix whose rows are all matrix-core rows (no lane
+    rows: an empty lane launch for it) next to one with lane rows only."""
+    from vbcsr_amd import workloads as W
+
+    K = kpm_api
+    Hs = [W.batched_hamiltonians(count=1, nb_lo=30, nb_hi=31, neighbours=6.0, seed=31, sizes=(9, 13))[0],
+          W.batched_hamiltonians(count=1, nb_lo=30, nb_hi=31, neighbours=6.0, seed=32, sizes=(1, 2))[0]]
+    mats = [mk(A) for A in Hs]
+    V0s = []
+    for h, H in enumerate(mats):
+        v = H.create_multivector(6)
+        v.set_random(500 + h, normalize=False)
+        V0s.append(v)
+    K.set_batch_plan(threshold, 2)
+    for graph in (False, True):
+        mu = K.kpm_moments_batched(mats, V0s, 16, mode="two", graph=graph)
+        for h in range(2):
+            ref = O.kpm_moments(O.Mat.of(Hs[h]), np.asarray(V0s[h].to_numpy()), 16, mode=1)
+            assert relerr(mu[h], ref) < 1e-11, (graph, h)
+    # only matrix-core rows in the whole batch: no lane launch at all
+    K.set_batch_plan(4, 1)
+    mu = K.kpm_moments_batched(mats[:1], V0s[:1], 16, mode="two")
+    ref = O.kpm_moments(O.Mat.of(Hs[0]), np.asarray(V0s[0].to_numpy()), 16, mode=1)
+    assert relerr(mu[0], ref) < 1e-11

@@ -108,3 +108,17 @@ def test_reduced_precision_numeric(numeric, tol):
     assert relerr(C.get_values(), Cr.vals) < tol
     D = dense(C)
     assert np.abs(D - D.T).max() == 0.0
+
+
+def test_hermitian_edge_cases():
+    """A threshold above every estimate gives an empty result; an empty matrix
+    gives an empty result."""
+    from vbcsr_amd import VBCSR
+    from vbcsr_amd import workloads as W
+
+    A = W.batched_hamiltonians(count=1, nb_lo=20, nb_hi=21, neighbours=4.0, seed=23)[0]
+    H = VBCSR.from_block_csr(A.bs, A.row_ptr, A.col_ind, A.vals)
+    C = H.spmm_hermitian(H, 1e30)
+    assert len(C.get_values()) == 0 and np.all(np.asarray(C.row_ptr) == 0)
+    E = VBCSR.from_block_csr(A.bs, np.zeros(A.nb + 1, np.int32), np.zeros(0, np.int32), np.zeros(0))
+    assert len(E.spmm_hermitian(E, 0.0).get_values()) == 0

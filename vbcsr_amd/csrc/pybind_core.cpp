@@ -14,6 +14,8 @@
 // when the host copy is newer; a buffer-protocol export (numpy view,
 // to_numpy) synchronises the host copy and marks it newer, so writes through
 // the view reach the device at the next device operation.
+#include <fstream>
+#include <iomanip>
 #include <pybind11/complex.h>
 #include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
@@ -619,6 +621,41 @@ struct Mat {
         auto res = std::make_shared<Mat>(graph_for_pattern(out, *graph), out);
         return res;
     }
+    // MatrixMarket coordinate export of the local matrix (block_csr.hpp:
+    // 2058-2131): header, "rows cols entries", then every stored scalar in
+    // slot order, row-major within a block, 1-based, values in scientific
+    // notation with 16 digits (complex: real and imaginary part).  One rank.
+    void save_matrix_market(const std::string& filename) {
+        std::ofstream file(filename);
+        if (!file.is_open()) throw std::runtime_error("Could not open file for writing: " + filename);
+        to_host();
+        constexpr bool cx = std::is_same_v<T, cplx>;
+        file << "%%MatrixMarket matrix coordinate " << (cx ? "complex" : "real") << " general\n";
+        const int n = graph->n_owned();
+        size_t nnz_e = 0;
+        for (int i = 0; i < n; ++i)
+            for (int k = graph->adj_ptr[i]; k < graph->adj_ptr[i + 1]; ++k)
+                nnz_e += size_t(graph->block_sizes[i]) * graph->block_sizes[graph->adj_ind[k]];
+        file << graph->owned_scalar_rows() << " " << graph->local_scalar_cols() << " " << nnz_e << "\n";
+        file << std::scientific << std::setprecision(16);
+        for (int i = 0; i < n; ++i) {
+            const int r = graph->block_sizes[i];
+            const int row1 = graph->block_offsets[i] + 1;
+            for (int k = graph->adj_ptr[i]; k < graph->adj_ptr[i + 1]; ++k) {
+                const int j = graph->adj_ind[k];
+                const int c = graph->block_sizes[j];
+                const int col1 = graph->block_offsets[j] + 1;
+                for (int a = 0; a < r; ++a)
+                    for (int b = 0; b < c; ++b) {
+                        const T v = host[voff[k] + int64_t(a) * c + b];
+                        file << (row1 + a) << " " << (col1 + b) << " ";
+                        if constexpr (cx) file << v.real() << " " << v.imag();
+                        else file << v;
+                        file << "\n";
+                    }
+            }
+        }
+    }
     // spmm_hermitian (block_csr.hpp:1597-1601): the upper block triangle of
     // op(A) B, then its Hermitian completion; spmm_hermitian_upper alone
     // (:1605-1616) and complete_hermitian (:1913-1920).
@@ -895,6 +932,7 @@ void bind_mat(py::module& m, const std::string& name) {
         .def("spmm_hermitian_upper", &M::spmm_hermitian_upper, py::arg("B"), py::arg("threshold"),
              py::arg("transA") = false, py::arg("numeric") = 0)
         .def("complete_hermitian", &M::complete_hermitian)
+        .def("save_matrix_market", &M::save_matrix_market)
         .def("spmm_self",
              [](M& self, double thr, bool transA) { return self.spmm(self, thr, transA, false, 0); },
              py::arg("threshold"), py::arg("transA") = false)

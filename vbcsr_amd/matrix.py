@@ -571,6 +571,11 @@ class VBCSR(LinearOperator):
         mat.assemble()
         return mat
 
+    def save_matrix_market(self, filename) -> None:
+        """MatrixMarket coordinate export of the local matrix (one rank),
+        as the reference writes it (block_csr.hpp:2058-2131)."""
+        self._core.save_matrix_market(str(filename))
+
     def to_scipy(self, format: Optional[str] = None) -> Any:
         import scipy.sparse as sp
 
