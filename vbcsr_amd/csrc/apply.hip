@@ -49,6 +49,17 @@ __global__ __launch_bounds__(kThreads) void k_apply_spmv(ApplyArgs<T> g) {
     }
 }
 
+// The fused SpMV step with the series accumulation (no dots).
+template <typename T, int RPP>
+__global__ __launch_bounds__(kThreads) void k_apply_spmv_series(ApplyArgs<T> g, SeriesAcc<T> sa) {
+    __shared__ __attribute__((aligned(16))) T tile[kWaves][kSpmvTile(RPP)];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int chunk = xcd_chunk(g.G);
+    double d1 = 0.0, d2 = 0.0;
+    row_loop_spmv<T, RPP, true, true>(g, g.split[chunk], g.split[chunk + 1], lane, wave, d1, d2, tile[wave], &sa);
+}
+
 RowConfig row_config(int K, int max_block, int uniform_bs) {
     RowConfig c{0, 0, 1};
     if (K == 1 && max_block <= 8) {
@@ -192,6 +203,26 @@ static void launch_single(const vbh_matrix* m, const ApplyArgs<T>& a, hipStream_
 }
 
 template <typename T, typename V>
+bool launch_spmv_series(const vbh_matrix* m, const ApplyArgs<T, V>& a, const SeriesAcc<T>& sa, hipStream_t s) {
+    if constexpr (is_mixed<T, V>()) {
+        return false;
+    } else {
+        if (uses_lanes(a) || a.partials != nullptr) return false;
+        const RowConfig c = row_config(a.K, m->max_block, m->uniform_bs);
+        if (c.spmv_rpp <= 0) return false;
+        const dim3 grid(a.G), block(kThreads);
+        switch (c.spmv_rpp) {
+            case 1: hipLaunchKernelGGL((k_apply_spmv_series<T, 1>), grid, block, 0, s, a, sa); break;
+            case 2: hipLaunchKernelGGL((k_apply_spmv_series<T, 2>), grid, block, 0, s, a, sa); break;
+            case 4: hipLaunchKernelGGL((k_apply_spmv_series<T, 4>), grid, block, 0, s, a, sa); break;
+            default: hipLaunchKernelGGL((k_apply_spmv_series<T, 8>), grid, block, 0, s, a, sa); break;
+        }
+        VBH_CHECK_HIP(hipGetLastError());
+        return true;
+    }
+}
+
+template <typename T, typename V>
 ApplyArgs<T, V> make_args(const vbh_matrix* m, const V* vals, int K, const T* X, int64_t ldx, T* Y, int64_t ldy) {
     ApplyArgs<T, V> a{};
     a.p = m->pattern();
@@ -257,7 +288,8 @@ const T* compute_values(vbh_matrix* m, hipStream_t s) {
     template void launch_rows<T, true, V>(const vbh_matrix*, const ApplyArgs<T, V>&, hipStream_t);              \
     template void launch_rows<T, false, V>(const vbh_matrix*, const ApplyArgs<T, V>&, hipStream_t);             \
     template ApplyArgs<T, V> make_args<T, V>(const vbh_matrix*, const V*, int, const T*, int64_t, T*, int64_t); \
-    template void apply_plain<T, V>(vbh_matrix*, const V*, int, const void*, int64_t, void*, int64_t, hipStream_t);
+    template void apply_plain<T, V>(vbh_matrix*, const V*, int, const void*, int64_t, void*, int64_t, hipStream_t); \
+    template bool launch_spmv_series<T, V>(const vbh_matrix*, const ApplyArgs<T, V>&, const SeriesAcc<T>&, hipStream_t);
 #define VBH_INST(T)                                                          \
     VBH_INST2(T, T)                                                          \
     template const T* compute_values<T>(vbh_matrix*, hipStream_t);          \

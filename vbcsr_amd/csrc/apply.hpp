@@ -102,10 +102,22 @@ __device__ inline int64_t uniform64(int64_t v) {
     return int64_t((uint64_t(hi) << 32) | lo);
 }
 
-// Fused epilogue for one output element (row yr, column v).
-template <typename T, bool FUSED>
+// Series accumulation for Chebyshev sums sum_n c_n t_n built term by term:
+// acc[row * ld + column] += (re + i im) * y for every stored (rounded) y of a
+// fused step.  A kernel parameter of its own, not an ApplyArgs field: a larger
+// argument struct raises the register use of every row kernel.
+template <typename T>
+struct SeriesAcc {
+    T* acc;
+    int64_t ld;
+    double re, im;
+};
+
+// Fused epilogue for one output element (row yr, column v); ACC: also the
+// series accumulation sa.
+template <typename T, bool FUSED, bool ACC = false>
 __device__ inline void emit(const ApplyArgs<T>& g, int64_t yr, int64_t xr, int v, typename Wide<T>::type acc,
-                            double& d1, double& d2) {
+                            double& d1, double& d2, const SeriesAcc<T>* sa = nullptr) {
     using A = typename Wide<T>::type;
     A y = acc;
     if constexpr (FUSED) {
@@ -119,6 +131,11 @@ __device__ inline void emit(const ApplyArgs<T>& g, int64_t yr, int64_t xr, int v
         d2 += redot(xo, ys);
     }
     g.Y[yr * g.ldy + v] = narrow_to<T>(y);
+    if constexpr (ACC) {
+        T* ap = sa->acc + yr * sa->ld + v;
+        const A c = Sc<A>::make(sa->re, sa->im);
+        *ap = narrow_to<T>(Sc<A>::fma(c, widen(narrow_to<T>(y)), widen(*ap)));
+    }
 }
 
 // SpMM row loop (see the file comment).  BS > 0: uniform BS x BS blocks.
@@ -297,9 +314,9 @@ __device__ __forceinline__ void spmv_issue(const ApplyArgs<T>& g, int64_t vrow, 
 // blocks holds at most 64*RPP values, plus up to two partial 16-byte words.
 // Software-pipelined within each 64-slot chunk of a row: run n+1's loads are
 // in flight while run n is computed from the tile.
-template <typename T, int RPP, bool FUSED>
+template <typename T, int RPP, bool FUSED, bool ACC = false>
 __device__ inline void row_loop_spmv(const ApplyArgs<T>& g, int r0, int r1, int lane, int wave, double& d1,
-                                     double& d2, T* lds) {
+                                     double& d2, T* lds, const SeriesAcc<T>* sa = nullptr) {
     using A = typename Wide<T>::type;
     constexpr int CH = 64 / RPP;  // blocks per staged run = partitions
     const DevPattern& p = g.p;
@@ -370,7 +387,8 @@ __device__ inline void row_loop_spmv(const ApplyArgs<T>& g, int r0, int r1, int 
         }
 #pragma unroll
         for (int s = RPP; s < 64; s <<= 1) acc = acc + shfl_xor(acc, s);
-        if (lane < r) emit<T, FUSED>(g, int64_t(p.roff[i]) + lane, int64_t(p.coff[i]) + lane, 0, acc, d1, d2);
+        if (lane < r)
+            emit<T, FUSED, ACC>(g, int64_t(p.roff[i]) + lane, int64_t(p.coff[i]) + lane, 0, acc, d1, d2, sa);
     }
 }
 
@@ -547,6 +565,13 @@ template <typename T, typename V = T>
 ApplyArgs<T, V> make_args(const vbh_matrix* m, const V* vals, int K, const T* X, int64_t ldx, T* Y, int64_t ldy);
 template <typename T>
 const T* compute_values(vbh_matrix* m, hipStream_t s);
+// One fused step through the K == 1 SpMV family with the series
+// accumulation sa; returns false (nothing launched) when that family does not
+// serve these arguments (lane family, blocks over 8, dots, mixed types): the
+// caller then launches launch_rows and accumulates separately.
+template <typename T, typename V>
+bool launch_spmv_series(const vbh_matrix* m, const ApplyArgs<T, V>& a, const SeriesAcc<T>& sa, hipStream_t s);
+
 template <typename T, typename V = T>
 void apply_plain(vbh_matrix* m, const V* vals, int K, const void* X, int64_t ldx, void* Y, int64_t ldy, hipStream_t s);
 template <typename T>

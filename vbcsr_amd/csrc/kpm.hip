@@ -711,8 +711,12 @@ static void cheb_apply(vbh_matrix* m, int K, const T* X, int ldx, T* Y, int ldy,
         a.gamma = st == 0 ? 0.0 : -1.0;
         a.Z = t[(st + 2) % 3];
         a.ldz = K;
-        launch_rows<T, true, V>(m, a, s);
-        cheb_accum<T>(N, K, t[(st + 1) % 3], Y, ldy, coef, n, s);
+        // Y += c_n t_n in the same launch where the SpMV family runs the step
+        const SeriesAcc<T> sa{Y, int64_t(ldy), coef[2 * n], coef[2 * n + 1]};
+        if (!launch_spmv_series<T, V>(m, a, sa, s)) {
+            launch_rows<T, true, V>(m, a, s);
+            cheb_accum<T>(N, K, t[(st + 1) % 3], Y, ldy, coef, n, s);
+        }
     }
 }
 
