@@ -16,8 +16,11 @@
 // * row_loop_spmv (K == 1, blocks up to 8 x 8): lanes are (row a, partition
 //   p); the values of each run of 64/RPP consecutive blocks (contiguous in
 //   memory) are first streamed into a wave-private LDS tile with 16-byte
-//   coalesced loads, then every lane forms its block-row dot product from LDS
-//   against x gathered from global memory.
+//   coalesced loads, each lane loads one x entry of the run into the tile's x
+//   strip, then every lane forms its block-row dot product from LDS.  The next
+//   run's loads are issued before the current run is computed.
+//
+// The lane family and the matrix-core class (lanes.hpp) serve 2 <= K <= 64.
 #pragma once
 
 #include <initializer_list>

@@ -1,17 +1,20 @@
-// Lane-per-row SpMM family for 2 <= K <= 64 (included by the per-type
-// lanes_*.hip translation units).
+// Lane-per-row SpMM family for 2 <= K <= 64 and the matrix-core class
+// (included by the per-type lanes_*.hip translation units).
 //
-// A wavefront owns 64/S consecutive scalar rows; lane l works on row
-// base + l/S and on the column slice [(l%S)*KT, (l%S)*KT + KT) of the K
-// right-hand sides.  Per block the lane reads its row of the block once (one
-// 16-byte load for 4x4 FP32 blocks) and, for every block column b, the KT
-// contiguous elements of X row (col + b) with vector loads, then issues KT
-// FMAs per loaded A element: all K columns share one A load and every lane
-// does useful work, where the column-per-lane map of row_loop spends most
-// of its issue slots on index and shuffle work at small K.  Lanes of one
-// block row read the same X rows (one cache line, broadcast).  Rows are
-// independent, so block rows of any size tile the wave; lanes of short
-// block rows simply finish their slot loop early.
+// Lane family: a wavefront owns R = 64/(S*P) consecutive scalar rows; lane l
+// works on row l % R, on the column slice (l / R) % S of KT columns of the K
+// right-hand sides, and on slot partition l / (R*S) of the row's blocks (the
+// partitions merge by xor shuffles).  Per block the lane reads its row of the
+// block once (one 16-byte load for 4x4 FP32 blocks) and, for every block
+// column b, the KT contiguous elements of X row (col + b) with vector loads,
+// then issues KT FMAs per loaded A element: all K columns share one A load.
+// For uniform 4x4 blocks the 4 rows of a block row are a lane quad that
+// shares the X rows by DPP broadcast.  Rows are independent, so block rows of
+// any size tile the wave; lanes of short block rows finish their loop early.
+//
+// Matrix-core class (mfma_rows): block rows of kMfmaMinRows..16 rows of real
+// matrices, one block row per wavefront on FP64 16x16x4 MFMA tiles (see the
+// section below).
 //
 // Accumulation is in FP64 registers for every storage precision.  The
 // epilogue is the fused one of row_loop: Y = alpha*AX + beta*X_own + gamma*Z
