@@ -381,9 +381,10 @@ struct Mat {
     uint32_t page_size_ = 0;
 
     explicit Mat(GraphP g) : graph(std::move(g)) { create_from_graph(); }
+    // The host mirror of a device-made matrix is allocated on first use
+    // (to_host): products and sums of large matrices never touch it.
     Mat(GraphP g, vbh_matrix_t handle) : graph(std::move(g)), h(handle) {
         compute_voff();
-        host.assign(nnz(), T(0));
         fresh = Fresh::Device;
     }
     ~Mat() {
@@ -417,8 +418,10 @@ struct Mat {
         }
         return h;
     }
+    // Invariant: host holds nnz() values unless fresh == Device.
     void to_host() {
         if (fresh == Fresh::Device) {
+            if (int64_t(host.size()) != nnz()) host.assign(nnz(), T(0));
             check(vbh_matrix_get_values(h, host.data(), 0, nullptr), "download values");
             fresh = Fresh::Both;
         }
@@ -585,7 +588,7 @@ struct Mat {
         h = m;
         graph = g;
         compute_voff();
-        host.assign(nnz(), T(0));
+        std::vector<T>().swap(host);
         fresh = Fresh::Device;
     }
     void axpby(T alpha, Mat& X, T beta) {

@@ -18,15 +18,19 @@
 // column, common.hpp:668) and builds C's pattern.
 //
 // Numeric phase: one work-group per row of C.  The A slots of the row are
-// visited in order; for each, the four waves take the B slots of row k
-// round-robin, locate C_ij by binary search in C's sorted row, and one lane
-// per element of the r x c output block accumulates sum_m A[a][m] B[m][b].
-// A work-group barrier separates consecutive A slots, so every C element is
+// visited in order.  Blocks up to 8 x 8 (k_numeric_owned): each C block is
+// owned by one wave, which scans the B slots of every row k lane-parallel and
+// looks C_ij up in an LDS column map of the row; no barriers.  Larger blocks
+// (k_numeric): the four waves take the B slots of row k round-robin, locate
+// C_ij by binary search in C's sorted row, and one lane per element of the
+// r x c output block accumulates sum_m A[a][m] B[m][b], with a work-group
+// barrier between consecutive A slots.  Either way every C element is
 // updated by one thread at a time in a fixed order -- deterministic, no
-// atomics.  numeric = 1 runs the block products in FP32 (FP32 accumulation,
-// FP64 result); numeric = 2 rounds A and B to BF16 and multiplies on the
-// CDNA4 matrix cores (v_mfma_f32_16x16x32_bf16, FP32 accumulation), one
-// 16-row panel of C_i per wave.
+// atomics, the same sums in the same order.  numeric = 1 runs the block
+// products in FP32 (FP32 accumulation, FP64 result); numeric = 2 rounds A and
+// B to BF16 and multiplies on the CDNA4 matrix cores
+// (v_mfma_f32_16x16x32_bf16, FP32 accumulation), one 16-row panel of C_i per
+// wave.
 #include <algorithm>
 #include <cstring>
 #include <numeric>
@@ -214,6 +218,94 @@ __global__ __launch_bounds__(256) void k_numeric(NumArgs<T, CT> a) {
             }
         }
         __syncthreads();
+    }
+    if (lane == 0 && used) atomicAdd(&a.npairs[0], used);
+}
+
+// Numeric phase for blocks up to 8 x 8 (the common case): C slot q of the row
+// belongs to wave q % 4, so the waves never touch each other's blocks and
+// run without barriers.  For every A slot each wave scans the B slots of row
+// k 64 at a time, one per lane: the skip test, C's slot of column j (from an
+// LDS map of the row when MAP, else a per-lane binary search), and the
+// offsets, all lane-parallel; the pairs the wave owns are compacted by a
+// ballot and processed one after another by lanes (a, b) = (lane / 8,
+// lane % 8), each holding its row of A_ik in registers.  Every C element
+// still receives its products in A-slot order and, within a product, in
+// inner-index order, exactly as in k_numeric.
+constexpr int kNumMapCols = 8192;  // LDS column map (nbc ints, dynamic) when nbc <= this
+
+template <typename T, typename CT, bool MAP>
+__global__ __launch_bounds__(256) void k_numeric_owned(NumArgs<T, CT> a, int nbc) {
+    extern __shared__ int colmap[];  // nbc entries when MAP
+    const int i = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int a0 = a.a_row_ptr[i], a1 = a.a_row_ptr[i + 1];
+    const int c0 = a.c_row_ptr[i], c1 = a.c_row_ptr[i + 1];
+    if (c0 == c1) return;
+    if constexpr (MAP) {
+        for (int j = tid; j < nbc; j += 256) colmap[j] = -1;
+        __syncthreads();
+        for (int q = c0 + tid; q < c1; q += 256) colmap[a.c_col_ind[q]] = q;
+        __syncthreads();
+    }
+    const int r = a.a_rbs[i];
+    const int row_count = a1 - a0;
+    const double row_eps = a.thr / double(row_count > 1 ? row_count : 1);
+    const int ra = lane >> 3, cb = lane & 7;
+    unsigned long long used = 0;
+    for (int ka = a0; ka < a1; ++ka) {
+        const int k = a.a_col_ind[ka];
+        const int m = a.a_cbs[k];
+        const double na = a.a_norm[ka];
+        const T* A = a.a_vals + a.a_voff[ka];
+        CT arow[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q)
+            arow[q] = (ra < r && q < m) ? narrow<T, CT>(A[ra * m + q]) : Sc<CT>::zero();
+        const int kb0 = a.b_row_ptr[k], kb1 = a.b_row_ptr[k + 1];
+        for (int base = kb0; base < kb1; base += 64) {
+            const int kb = base + lane;
+            int pos = -1, j = 0;
+            if (kb < kb1) {
+                j = a.b_col_ind[kb];
+                if (!(na * a.b_norm[kb] < row_eps)) {
+                    if constexpr (MAP) {
+                        pos = colmap[j];
+                    } else {
+                        const int p = find_col(a.c_col_ind, c0, c1, j);
+                        pos = (p < c1 && a.c_col_ind[p] == j) ? p : -1;
+                    }
+                }
+            }
+            const bool mine = pos >= 0 && ((pos - c0) & 3) == wave;
+            uint64_t mask = __ballot(mine);
+            used += __popcll(mask);
+            int64_t boff = 0, coff = 0;
+            int cw = 0;
+            if (mine) {
+                boff = a.b_voff[kb];
+                coff = a.c_voff[pos];
+                cw = a.c_cbs[j];
+            }
+            while (mask) {
+                const int t = __builtin_ctzll(mask);
+                mask &= mask - 1;
+                const int c = __builtin_amdgcn_readlane(cw, t);
+                const int64_t bo = int64_t((uint64_t(uint32_t(__builtin_amdgcn_readlane(int(boff >> 32), t))) << 32) |
+                                           uint32_t(__builtin_amdgcn_readlane(int(boff), t)));
+                const int64_t co = int64_t((uint64_t(uint32_t(__builtin_amdgcn_readlane(int(coff >> 32), t))) << 32) |
+                                           uint32_t(__builtin_amdgcn_readlane(int(coff), t)));
+                if (ra < r && cb < c) {
+                    const T* B = a.b_vals + bo + cb;
+                    T* C = a.c_vals + co + ra * c + cb;
+                    CT acc = narrow<T, CT>(*C);
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (q < m) acc = Sc<CT>::fma(arow[q], narrow<T, CT>(B[int64_t(q) * c]), acc);
+                    *C = widen<T, CT>(acc);
+                }
+            }
+        }
     }
     if (lane == 0 && used) atomicAdd(&a.npairs[0], used);
 }
@@ -509,7 +601,15 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
                     NumArgs<T, CT> na{A->row_ptr, A->col_ind, A->voff, (const T*)A->vals, an, A->rbs, A->cbs,
                                       B->row_ptr, B->col_ind, B->voff, (const T*)B->vals, bn, C->row_ptr,
                                       C->col_ind, C->voff, C->cbs, (T*)C->vals, d_np, thr};
-                    hipLaunchKernelGGL((k_numeric<T, CT>), dim3(nbr), dim3(256), 0, s, na);
+                    if (A->max_block <= 8 && B->max_block <= 8) {
+                        if (C->nbc <= kNumMapCols)
+                            hipLaunchKernelGGL((k_numeric_owned<T, CT, true>), dim3(nbr), dim3(256),
+                                               sizeof(int) * size_t(C->nbc), s, na, C->nbc);
+                        else
+                            hipLaunchKernelGGL((k_numeric_owned<T, CT, false>), dim3(nbr), dim3(256), 0, s, na, C->nbc);
+                    } else {
+                        hipLaunchKernelGGL((k_numeric<T, CT>), dim3(nbr), dim3(256), 0, s, na);
+                    }
                     VBH_CHECK_HIP(hipGetLastError());
                 };
                 if (A->dtype == VBH_F64) {
