@@ -287,22 +287,44 @@ __global__ __launch_bounds__(256) void k_numeric_owned(NumArgs<T, CT> a, int nbc
                 coff = a.c_voff[pos];
                 cw = a.c_cbs[j];
             }
+            // UP owned pairs per pass: all their loads are issued before the
+            // first product (indices clamped into the blocks, so every load
+            // is unconditional and in bounds); the pairs of one pass have
+            // distinct C blocks, and lane (a, b) always owns element (a, b)
+            constexpr int UP = 2;  // measured on McWeeny products: 2 beats 1 (13 %) and 4 (3 %)
+            const int rac = ra < r ? ra : r - 1;
             while (mask) {
-                const int t = __builtin_ctzll(mask);
+                int tt[UP];
+                tt[0] = __builtin_ctzll(mask);
                 mask &= mask - 1;
-                const int c = __builtin_amdgcn_readlane(cw, t);
-                const int64_t bo = int64_t((uint64_t(uint32_t(__builtin_amdgcn_readlane(int(boff >> 32), t))) << 32) |
-                                           uint32_t(__builtin_amdgcn_readlane(int(boff), t)));
-                const int64_t co = int64_t((uint64_t(uint32_t(__builtin_amdgcn_readlane(int(coff >> 32), t))) << 32) |
-                                           uint32_t(__builtin_amdgcn_readlane(int(coff), t)));
-                if (ra < r && cb < c) {
-                    const T* B = a.b_vals + bo + cb;
-                    T* C = a.c_vals + co + ra * c + cb;
-                    CT acc = narrow<T, CT>(*C);
+#pragma unroll
+                for (int u = 1; u < UP; ++u) {
+                    tt[u] = mask ? __builtin_ctzll(mask) : -1;
+                    if (mask) mask &= mask - 1;
+                }
+                int cc[UP];
+                int64_t bo[UP], co[UP];
+                CT acc[UP], bv[UP][8];
+#pragma unroll
+                for (int u = 0; u < UP; ++u) {
+                    const int t = tt[u] >= 0 ? tt[u] : tt[0];  // a missing pair repeats pair 0 (not stored)
+                    cc[u] = __builtin_amdgcn_readlane(cw, t);
+                    bo[u] = int64_t((uint64_t(uint32_t(__builtin_amdgcn_readlane(int(boff >> 32), t))) << 32) |
+                                    uint32_t(__builtin_amdgcn_readlane(int(boff), t)));
+                    co[u] = int64_t((uint64_t(uint32_t(__builtin_amdgcn_readlane(int(coff >> 32), t))) << 32) |
+                                    uint32_t(__builtin_amdgcn_readlane(int(coff), t)));
+                    const int cbc = cb < cc[u] ? cb : cc[u] - 1;
+                    acc[u] = narrow<T, CT>(a.c_vals[co[u] + rac * cc[u] + cbc]);
 #pragma unroll
                     for (int q = 0; q < 8; ++q)
-                        if (q < m) acc = Sc<CT>::fma(arow[q], narrow<T, CT>(B[int64_t(q) * c]), acc);
-                    *C = widen<T, CT>(acc);
+                        bv[u][q] = q < m ? narrow<T, CT>(a.b_vals[bo[u] + int64_t(q) * cc[u] + cbc]) : Sc<CT>::zero();
+                }
+#pragma unroll
+                for (int u = 0; u < UP; ++u) {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q)
+                        if (q < m) acc[u] = Sc<CT>::fma(arow[q], bv[u][q], acc[u]);
+                    if (tt[u] >= 0 && ra < r && cb < cc[u]) a.c_vals[co[u] + ra * cc[u] + cb] = widen<T, CT>(acc[u]);
                 }
             }
         }
