@@ -118,3 +118,25 @@ def test_mcweeny_iterations_vs_oracle():
         assert np.array_equal(P.row_ptr, pm.row_ptr) and np.array_equal(P.col_ind, pm.col_ind), it
         assert relerr(P.get_values(), pm.vals) < 1e-11, it
     assert abs(P.trace() - np.trace(W.BlockCSR(P0.bs, pm.row_ptr, pm.col_ind, pm.vals).to_dense())) < 1e-8
+
+
+@pytest.mark.parametrize("thr", [0.0, 0.05, -1.0])
+def test_wide_rows_symbolic_paths(thr):
+    """Rows whose hash table would exceed the LDS classes (min(paths, nbc) >
+    2047): with thr >= 0 they take the dense per-column estimate kernel
+    (columns emitted in order), with thr < 0 the global hash tables (every
+    reached column kept, est > thr).  Both against the oracle: 2200 block
+    rows of sizes 1-2, 47 blocks per row, so about 2200 rows have 2209 paths."""
+    from vbcsr_amd import VBCSR
+    from vbcsr_amd import workloads as W
+
+    A = W.random_vbcsr(2200, bs_lo=1, bs_hi=2, per_row=46, seed=3)
+    rows = np.diff(A.row_ptr)
+    paths = np.array([rows[A.col_ind[A.row_ptr[i]:A.row_ptr[i + 1]]].sum() for i in range(A.nb)])
+    assert (np.minimum(paths, A.nb) > 2047).sum() > 2000
+    H = VBCSR.from_block_csr(A.bs, A.row_ptr, A.col_ind, A.vals)
+    m = O.Mat.of(A)
+    C = H.spmm(H, thr)
+    Cr = O.spgemm(m, m, thr)
+    assert np.array_equal(C.row_ptr, Cr.row_ptr) and np.array_equal(C.col_ind, Cr.col_ind)
+    assert relerr(C.get_values(), Cr.vals) < 1e-12

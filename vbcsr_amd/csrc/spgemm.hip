@@ -134,6 +134,56 @@ __global__ __launch_bounds__(256) void k_symbolic(SymArgs a) {
     if (threadIdx.x == 0) a.out_count[i] = cnt;
 }
 
+// Symbolic phase for rows whose hash table would not fit in LDS, when C has
+// nbc <= kNumMapCols block columns and thr >= 0: the estimates live in a
+// dense per-column FP64 array in LDS (no hashing, no global tables).  Column j
+// is kept iff est_j > thr, which for thr >= 0 also drops every column no path
+// reached (est 0), as the hash table's absent keys are.  The kept columns are
+// written in ascending order (a block-wide scan over per-thread column
+// ranges), so the host sort finds the row already sorted.
+__global__ __launch_bounds__(256) void k_symbolic_dense(SymArgs a, int nbc) {
+    extern __shared__ double est[];  // [nbc]
+    __shared__ int scan[256];
+    const int q = blockIdx.x;
+    if (q >= a.nrows) return;
+    const int i = a.rows[q];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int t = tid; t < nbc; t += 256) est[t] = 0.0;
+    __syncthreads();
+    int64_t npairs = 0;
+    for (int ka = a.a_row_ptr[i] + wave; ka < a.a_row_ptr[i + 1]; ka += 4) {
+        const int k = a.a_col_ind[ka];
+        const double na = a.a_norm[ka];
+        const int b0 = a.b_row_ptr[k], b1 = a.b_row_ptr[k + 1];
+        npairs += b1 - b0;
+        for (int kb = b0 + lane; kb < b1; kb += 64) {
+            const int j = a.b_col_ind[kb];
+            if (a.upper && j < i) continue;  // common.hpp:625 (upper_only)
+            atomicAdd(&est[j], na * a.b_norm[kb]);
+        }
+    }
+    __syncthreads();
+    // thread t owns columns [t*C, (t+1)*C): count, exclusive scan, write in order
+    const int C = (nbc + 255) / 256;
+    const int j0 = tid * C, j1 = j0 + C < nbc ? j0 + C : nbc;
+    int mine = 0;
+    for (int j = j0; j < j1; ++j) mine += est[j] > a.thr;
+    scan[tid] = mine;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int v = tid >= off ? scan[tid - off] : 0;
+        __syncthreads();
+        scan[tid] += v;
+        __syncthreads();
+    }
+    int* out = a.out_cols + a.out_off[i];
+    int pos = scan[tid] - mine;
+    for (int j = j0; j < j1; ++j)
+        if (est[j] > a.thr) out[pos++] = j;
+    if (lane == 0) atomicAdd((unsigned long long*)&a.pairs[i], (unsigned long long)npairs);
+    if (tid == 255) a.out_count[i] = scan[255];
+}
+
 template <typename T, typename CT>
 struct NumArgs {
     const int* a_row_ptr;
@@ -634,7 +684,15 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
             hipLaunchKernelGGL(k_symbolic<true>, dim3(sa.nrows), dim3(256), shm, s, sa);
             VBH_CHECK_HIP(hipGetLastError());
         }
-        if (!rows_cls[3].empty()) {
+        if (!rows_cls[3].empty() && B->nbc <= kNumMapCols && thr >= 0.0) {
+            int* d_rows = upload(rows_cls[3], s);
+            tmp_free.push_back(d_rows);
+            sa.rows = d_rows;
+            sa.nrows = int(rows_cls[3].size());
+            hipLaunchKernelGGL(k_symbolic_dense, dim3(sa.nrows), dim3(256), sizeof(double) * size_t(B->nbc), s, sa,
+                               B->nbc);
+            VBH_CHECK_HIP(hipGetLastError());
+        } else if (!rows_cls[3].empty()) {
             // global tables, processed in batches bounded to ~1 GiB of tables
             const auto& rows = rows_cls[3];
             size_t pos = 0;
@@ -679,7 +737,8 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
         cci.reserve(total);
         for (int i = 0; i < nbr; ++i) {
             const int64_t b = out_off[i];
-            std::sort(cols.begin() + b, cols.begin() + b + counts[i]);
+            if (!std::is_sorted(cols.begin() + b, cols.begin() + b + counts[i]))
+                std::sort(cols.begin() + b, cols.begin() + b + counts[i]);
             cci.insert(cci.end(), cols.begin() + b, cols.begin() + b + counts[i]);
             crp[i + 1] = int(cci.size());
         }
