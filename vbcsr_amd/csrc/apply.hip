@@ -224,6 +224,10 @@ bool launch_spmv_series(const vbh_matrix* m, const ApplyArgs<T, V>& a, const Ser
 
 template <typename T, typename V>
 ApplyArgs<T, V> make_args(const vbh_matrix* m, const V* vals, int K, const T* X, int64_t ldx, T* Y, int64_t ldy) {
+    // row-class lists of the active matrix-core threshold, built on first use
+    // (before any group count is derived from them; never inside a capture:
+    // the captured batched launches select their lists beforehand)
+    select_row_classes(const_cast<vbh_matrix*>(m), m->mfma_min_rows, nullptr);
     ApplyArgs<T, V> a{};
     a.p = m->pattern();
     a.parts = 1;

@@ -355,7 +355,10 @@ vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const
         }
         m->h_roff = roff;
         m->h_coff = coff;
-        select_row_classes(m, kMfmaMinRows, s);
+        // the row-class lists (breadth-first order, matrix-core descriptors) are
+        // built by the first apply (make_args): products, filtered copies and
+        // sums that are never applied to vectors do not pay for them
+        m->mfma_min_rows = kMfmaMinRows;
         m->split = upload(split, s);
         // kVecPad trailing elements: the streaming kernels read whole 16-byte
         // words, which may reach up to one word past the last value
