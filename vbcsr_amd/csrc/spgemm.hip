@@ -565,6 +565,9 @@ __global__ __launch_bounds__(256) void k_numeric_mfma_bf16_map(MfmaArgs a, int n
                         const int2 e = bmap[jcol[t]];
                         if (e.x == ka) Bcol = a.b_vals + a.b_voff[e.y] + cb[t];
                     }
+                    // a strip with no B block of this A slot would add exact
+                    // zeros: the wave skips its loads and product
+                    if (__ballot(Bcol != nullptr) == 0) continue;
                     bf16x8 bv;
 #pragma unroll
                     for (int e = 0; e < 8; ++e) {
