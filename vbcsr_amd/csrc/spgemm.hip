@@ -205,6 +205,11 @@ struct NumArgs {
     T* c_vals;
     unsigned long long* npairs;
     double thr;
+    // CT != T (FP32 products) in k_numeric_owned: A and B from their CT value
+    // copies, C accumulated in a CT array (widened into c_vals afterwards)
+    const CT* a_c;
+    const CT* b_c;
+    CT* c_c;
 };
 
 __device__ inline int find_col(const int* __restrict__ cols, int lo, int hi, int j) {
@@ -286,6 +291,7 @@ constexpr int kNumMapCols = 8192;  // LDS column map (nbc ints, dynamic) when nb
 
 template <typename T, typename CT, bool MAP>
 __global__ __launch_bounds__(256) void k_numeric_owned(NumArgs<T, CT> a, int nbc) {
+    constexpr bool NS = !std::is_same_v<T, CT>;  // narrow storage: a_c, b_c, c_c
     extern __shared__ int colmap[];  // nbc entries when MAP
     const int i = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -309,9 +315,15 @@ __global__ __launch_bounds__(256) void k_numeric_owned(NumArgs<T, CT> a, int nbc
         const double na = a.a_norm[ka];
         const T* A = a.a_vals + a.a_voff[ka];
         CT arow[8];
+        if constexpr (NS) {
+            const CT* Ac = a.a_c + a.a_voff[ka];
 #pragma unroll
-        for (int q = 0; q < 8; ++q)
-            arow[q] = (ra < r && q < m) ? narrow<T, CT>(A[ra * m + q]) : Sc<CT>::zero();
+            for (int q = 0; q < 8; ++q) arow[q] = (ra < r && q < m) ? Ac[ra * m + q] : Sc<CT>::zero();
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+                arow[q] = (ra < r && q < m) ? narrow<T, CT>(A[ra * m + q]) : Sc<CT>::zero();
+        }
         const int kb0 = a.b_row_ptr[k], kb1 = a.b_row_ptr[k + 1];
         for (int base = kb0; base < kb1; base += 64) {
             const int kb = base + lane;
@@ -364,22 +376,39 @@ __global__ __launch_bounds__(256) void k_numeric_owned(NumArgs<T, CT> a, int nbc
                     co[u] = int64_t((uint64_t(uint32_t(__builtin_amdgcn_readlane(int(coff >> 32), t))) << 32) |
                                     uint32_t(__builtin_amdgcn_readlane(int(coff), t)));
                     const int cbc = cb < cc[u] ? cb : cc[u] - 1;
-                    acc[u] = narrow<T, CT>(a.c_vals[co[u] + rac * cc[u] + cbc]);
+                    if constexpr (NS) {
+                        acc[u] = a.c_c[co[u] + rac * cc[u] + cbc];
 #pragma unroll
-                    for (int q = 0; q < 8; ++q)
-                        bv[u][q] = q < m ? narrow<T, CT>(a.b_vals[bo[u] + int64_t(q) * cc[u] + cbc]) : Sc<CT>::zero();
+                        for (int q = 0; q < 8; ++q)
+                            bv[u][q] = q < m ? a.b_c[bo[u] + int64_t(q) * cc[u] + cbc] : Sc<CT>::zero();
+                    } else {
+                        acc[u] = narrow<T, CT>(a.c_vals[co[u] + rac * cc[u] + cbc]);
+#pragma unroll
+                        for (int q = 0; q < 8; ++q)
+                            bv[u][q] = q < m ? narrow<T, CT>(a.b_vals[bo[u] + int64_t(q) * cc[u] + cbc]) : Sc<CT>::zero();
+                    }
                 }
 #pragma unroll
                 for (int u = 0; u < UP; ++u) {
 #pragma unroll
                     for (int q = 0; q < 8; ++q)
                         if (q < m) acc[u] = Sc<CT>::fma(arow[q], bv[u][q], acc[u]);
-                    if (tt[u] >= 0 && ra < r && cb < cc[u]) a.c_vals[co[u] + ra * cc[u] + cb] = widen<T, CT>(acc[u]);
+                    if (tt[u] >= 0 && ra < r && cb < cc[u]) {
+                        if constexpr (NS) a.c_c[co[u] + ra * cc[u] + cb] = acc[u];
+                        else a.c_vals[co[u] + ra * cc[u] + cb] = widen<T, CT>(acc[u]);
+                    }
                 }
             }
         }
     }
     if (lane == 0 && used) atomicAdd(&a.npairs[0], used);
+}
+
+// c_vals = widen(c_c) after an FP32-product numeric phase.
+template <typename T, typename CT>
+__global__ void k_widen_values(const CT* __restrict__ in, T* __restrict__ out, int64_t n) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = widen<T, CT>(in[i]);
 }
 
 // ------------------------------------------------------------ BF16 MFMA numeric
@@ -780,13 +809,33 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
                     using CT = std::remove_pointer_t<decltype(ctag)>;
                     NumArgs<T, CT> na{A->row_ptr, A->col_ind, A->voff, (const T*)A->vals, an, A->rbs, A->cbs,
                                       B->row_ptr, B->col_ind, B->voff, (const T*)B->vals, bn, C->row_ptr,
-                                      C->col_ind, C->voff, C->cbs, (T*)C->vals, d_np, thr};
+                                      C->col_ind, C->voff, C->cbs, (T*)C->vals, d_np, thr,
+                                      nullptr, nullptr, nullptr};
                     if (A->max_block <= 8 && B->max_block <= 8) {
+                        constexpr bool narrow_products = !std::is_same_v<T, CT>;
+                        if constexpr (narrow_products) {
+                            // FP32 products from the FP32 value copies (rounded as
+                            // the products round them), accumulated in FP32
+                            ensure_vals32(A, s);
+                            ensure_vals32(B, s);
+                            na.a_c = (const CT*)A->vals32;
+                            na.b_c = (const CT*)B->vals32;
+                            na.c_c = (CT*)dalloc(sizeof(CT) * size_t(std::max<int64_t>(C->nnz, 1)));
+                            VBH_CHECK_HIP(hipMemsetAsync(na.c_c, 0, sizeof(CT) * size_t(C->nnz), s));
+                        }
                         if (C->nbc <= kNumMapCols)
                             hipLaunchKernelGGL((k_numeric_owned<T, CT, true>), dim3(nbr), dim3(256),
                                                sizeof(int) * size_t(C->nbc), s, na, C->nbc);
                         else
                             hipLaunchKernelGGL((k_numeric_owned<T, CT, false>), dim3(nbr), dim3(256), 0, s, na, C->nbc);
+                        if constexpr (narrow_products) {
+                            VBH_CHECK_HIP(hipGetLastError());
+                            hipLaunchKernelGGL((k_widen_values<T, CT>), dim3(unsigned((C->nnz + 255) / 256)), dim3(256),
+                                               0, s, na.c_c, (T*)C->vals, C->nnz);
+                            VBH_CHECK_HIP(hipGetLastError());
+                            VBH_CHECK_HIP(hipStreamSynchronize(s));
+                            dfree(na.c_c);
+                        }
                     } else {
                         hipLaunchKernelGGL((k_numeric<T, CT>), dim3(nbr), dim3(256), 0, s, na);
                     }
