@@ -140,3 +140,22 @@ def test_wide_rows_symbolic_paths(thr):
     Cr = O.spgemm(m, m, thr)
     assert np.array_equal(C.row_ptr, Cr.row_ptr) and np.array_equal(C.col_ind, Cr.col_ind)
     assert relerr(C.get_values(), Cr.vals) < 1e-12
+
+
+@pytest.mark.parametrize("numeric,tol", [("fp64", 1e-12), ("fp32", 1e-5), ("bf16", 2e-2)])
+def test_many_block_columns_numeric(numeric, tol):
+    """More than 8192 block columns: no LDS column map, so the FP64/FP32
+    numeric phase locates C blocks by a per-lane binary search and BF16 uses
+    the binary-search strip kernel.  9000 block rows of sizes 1-3, 9 blocks
+    per row, against the oracle."""
+    from vbcsr_amd import VBCSR
+    from vbcsr_amd import workloads as W
+
+    A = W.random_vbcsr(9000, bs_lo=1, bs_hi=3, per_row=8, seed=11)
+    assert A.nb > 8192
+    H = VBCSR.from_block_csr(A.bs, A.row_ptr, A.col_ind, A.vals)
+    m = O.Mat.of(A)
+    C = H.spmm(H, 0.0, numeric=numeric)
+    Cr = O.spgemm(m, m, 0.0)
+    assert np.array_equal(C.row_ptr, Cr.row_ptr) and np.array_equal(C.col_ind, Cr.col_ind)
+    assert relerr(C.get_values(), Cr.vals) < tol
