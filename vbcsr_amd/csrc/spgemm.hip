@@ -184,6 +184,18 @@ __global__ __launch_bounds__(256) void k_symbolic_dense(SymArgs a, int nbc) {
     if (tid == 255) a.out_count[i] = scan[255];
 }
 
+// Row i's kept columns, out_cols[out_off[i] .. + (crp[i+1] - crp[i])), to
+// cci[crp[i] ..]: one work-group per row.
+__global__ __launch_bounds__(256) void k_compact_cols(const int* __restrict__ out_cols,
+                                                      const int64_t* __restrict__ out_off,
+                                                      const int* __restrict__ crp, int* __restrict__ cci) {
+    const int i = blockIdx.x;
+    const int n = crp[i + 1] - crp[i];
+    const int* src = out_cols + out_off[i];
+    int* dst = cci + crp[i];
+    for (int t = threadIdx.x; t < n; t += 256) dst[t] = src[t];
+}
+
 template <typename T, typename CT>
 struct NumArgs {
     const int* a_row_ptr;
@@ -751,29 +763,36 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
                 for (void* p : {(void*)d_rows, (void*)d_goff, (void*)gk, (void*)gv}) dfree(p);
             }
         }
-        // ---- pattern of C (host sort per row)
+        // ---- pattern of C: the per-row kept columns are compacted on the
+        // device (row i's count at crp[i]), so only they cross to the host,
+        // which sorts the rows the hash tables left unsorted
         std::vector<int> counts(nbr);
-        std::vector<int> cols(out_off[nbr]);
         std::vector<int64_t> pairs(nbr);
         VBH_CHECK_HIP(hipMemcpyAsync(counts.data(), d_count, sizeof(int) * nbr, hipMemcpyDeviceToHost, s));
-        if (!cols.empty())
-            VBH_CHECK_HIP(hipMemcpyAsync(cols.data(), d_out_cols, sizeof(int) * cols.size(), hipMemcpyDeviceToHost, s));
         VBH_CHECK_HIP(hipMemcpyAsync(pairs.data(), d_pairs, sizeof(int64_t) * nbr, hipMemcpyDeviceToHost, s));
         VBH_CHECK_HIP(hipStreamSynchronize(s));
+        std::vector<int> crp(nbr + 1, 0);
+        for (int i = 0; i < nbr; ++i) {
+            VBH_REQUIRE(int64_t(crp[i]) + counts[i] <= int64_t(INT32_MAX), "SpGEMM result exceeds 2^31 blocks");
+            crp[i + 1] = crp[i] + counts[i];
+        }
+        const int64_t total = crp[nbr];
+        std::vector<int> cci(total);
+        if (total > 0) {
+            int* d_crp = upload(crp, s);
+            int* d_cci = (int*)dalloc(sizeof(int) * size_t(total));
+            hipLaunchKernelGGL(k_compact_cols, dim3(nbr), dim3(256), 0, s, d_out_cols, d_out_off, d_crp, d_cci);
+            VBH_CHECK_HIP(hipGetLastError());
+            VBH_CHECK_HIP(hipMemcpyAsync(cci.data(), d_cci, sizeof(int) * size_t(total), hipMemcpyDeviceToHost, s));
+            VBH_CHECK_HIP(hipStreamSynchronize(s));
+            dfree(d_crp);
+            dfree(d_cci);
+        }
         for (void* p : tmp_free) dfree(p);
         for (void* p : {(void*)d_tsize, (void*)d_out_off, (void*)d_out_cols, (void*)d_count, (void*)d_pairs}) dfree(p);
-        std::vector<int> crp(nbr + 1, 0), cci;
-        int64_t total = 0;
-        for (int i = 0; i < nbr; ++i) total += counts[i];
-        VBH_REQUIRE(total <= int64_t(INT32_MAX), "SpGEMM result exceeds 2^31 blocks");
-        cci.reserve(total);
-        for (int i = 0; i < nbr; ++i) {
-            const int64_t b = out_off[i];
-            if (!std::is_sorted(cols.begin() + b, cols.begin() + b + counts[i]))
-                std::sort(cols.begin() + b, cols.begin() + b + counts[i]);
-            cci.insert(cci.end(), cols.begin() + b, cols.begin() + b + counts[i]);
-            crp[i + 1] = int(cci.size());
-        }
+        for (int i = 0; i < nbr; ++i)
+            if (!std::is_sorted(cci.begin() + crp[i], cci.begin() + crp[i + 1]))
+                std::sort(cci.begin() + crp[i], cci.begin() + crp[i + 1]);
         vbh_matrix* C = build_matrix(nbr, B->nbc, A->h_rbs.data(), B->h_cbs.data(), crp.data(), cci.data(), A->dtype, s);
         unsigned long long* d_np = (unsigned long long*)dalloc(sizeof(unsigned long long));
         VBH_CHECK_HIP(hipMemsetAsync(d_np, 0, sizeof(unsigned long long), s));
