@@ -605,6 +605,26 @@ struct Mat {
         if (out) adopt(out);
         else device_written();
     }
+    // alpha*X + beta*self as a new matrix, self unchanged: a union of two
+    // patterns is built once, without first duplicating self
+    std::shared_ptr<Mat> combined(T alpha, Mat& X, T beta) {
+        const bool same_pattern = this == &X || (X.graph->adj_ptr == graph->adj_ptr &&
+                                                 X.graph->adj_ind == graph->adj_ind &&
+                                                 X.graph->block_sizes == graph->block_sizes);
+        if (same_pattern) {
+            auto c = duplicate(true);
+            c->axpby(alpha, this == &X ? *c : X, beta);
+            return c;
+        }
+        if (!X.graph->same_owned(*graph)) throw std::runtime_error("Matrix row count mismatch in axpby");
+        double a[2], b[2];
+        scalar2(alpha, a);
+        scalar2(beta, b);
+        vbh_matrix_t out = nullptr;
+        check(vbh_matrix_axpby(dev(), a, X.dev(), b, &out, nullptr), "axpby");
+        if (!out) throw std::runtime_error("axpby: expected a union result");
+        return std::make_shared<Mat>(graph_for(out), out);
+    }
     void filter_blocks(double thr) {
         vbh_matrix_t out = nullptr;
         check(vbh_matrix_filter(dev(), thr, &out, nullptr), "filter_blocks");
@@ -925,6 +945,7 @@ void bind_mat(py::module& m, const std::string& name) {
         .def("add_diagonal", &M::add_diagonal)
         .def("axpy", [](M& self, T alpha, M& X) { self.axpby(alpha, X, T(1)); })
         .def("axpby", &M::axpby)
+        .def("combined", &M::combined, py::arg("alpha"), py::arg("X"), py::arg("beta"))
         .def("copy_from", &M::copy_from)
         .def("fill", &M::fill)
         .def("duplicate", &M::duplicate, py::arg("independent_graph") = true)

@@ -389,16 +389,14 @@ class VBCSR(LinearOperator):
 
     def __add__(self, other):
         if isinstance(other, VBCSR):
-            res = self.duplicate()
-            res += other
-            return res
+            return self._wrap_core(self._core.combined(1.0, other._core, 1.0), self.dtype, self.comm,
+                                   shape=self.shape)
         return NotImplemented
 
     def __sub__(self, other):
         if isinstance(other, VBCSR):
-            res = self.duplicate()
-            res -= other
-            return res
+            return self._wrap_core(self._core.combined(-1.0, other._core, 1.0), self.dtype, self.comm,
+                                   shape=self.shape)
         return NotImplemented
 
     def __isub__(self, other):
