@@ -561,15 +561,64 @@ VBH_API int vbh_matrix_get_values(vbh_matrix_t m, void* vals, int location, void
 
 VBH_API void* vbh_matrix_values(vbh_matrix_t m) { return m ? m->vals : nullptr; }
 
+// A copy of m: the host pattern, device-to-device copies of the device
+// pattern arrays and values (nothing is recomputed on the host); the FP32
+// copy, row-class lists and adjoint plan are rebuilt on demand.
+static vbh_matrix* clone_matrix(const vbh_matrix* m, hipStream_t s) {
+    auto* c = new vbh_matrix();
+    try {
+        c->nbr = m->nbr;
+        c->nbc = m->nbc;
+        c->nnzb = m->nnzb;
+        c->nnz = m->nnz;
+        c->n_rows = m->n_rows;
+        c->n_cols = m->n_cols;
+        c->dtype = m->dtype;
+        c->kind = m->kind;
+        c->max_block = m->max_block;
+        c->n_classes = m->n_classes;
+        c->uniform_bs = m->uniform_bs;
+        c->h_rbs = m->h_rbs;
+        c->h_cbs = m->h_cbs;
+        c->h_row_ptr = m->h_row_ptr;
+        c->h_col_ind = m->h_col_ind;
+        c->h_voff = m->h_voff;
+        c->h_roff = m->h_roff;
+        c->h_coff = m->h_coff;
+        c->h_bfs = m->h_bfs;
+        c->mfma_min_rows = m->mfma_min_rows;
+        c->G = m->G;
+        auto copy = [&](auto* src, int64_t count) {
+            using E = std::remove_pointer_t<decltype(src)>;
+            E* d = static_cast<E*>(dalloc(sizeof(E) * size_t(count > 0 ? count : 1)));
+            if (count > 0) VBH_CHECK_HIP(hipMemcpyAsync(d, src, sizeof(E) * size_t(count), hipMemcpyDeviceToDevice, s));
+            return d;
+        };
+        c->rbs = copy(m->rbs, m->nbr);
+        c->cbs = copy(m->cbs, m->nbc);
+        c->row_ptr = copy(m->row_ptr, int64_t(m->nbr) + 1);
+        c->col_ind = copy(m->col_ind, m->nnzb);
+        c->roff = copy(m->roff, int64_t(m->nbr) + 1);
+        c->coff = copy(m->coff, int64_t(m->nbc) + 1);
+        c->voff = copy(m->voff, m->nnzb + 1);
+        c->meta = copy(m->meta, m->nnzb);
+        c->row_of = copy(m->row_of, m->n_rows > 0 ? m->n_rows : 1);
+        c->split = copy(m->split, int64_t(m->G) + 1);
+        const size_t vbytes = m->elem_size() * size_t(m->nnz + kVecPad);
+        c->vals = dalloc(vbytes);
+        VBH_CHECK_HIP(hipMemcpyAsync(c->vals, m->vals, vbytes, hipMemcpyDeviceToDevice, s));  // padding included
+        VBH_CHECK_HIP(hipStreamSynchronize(s));
+    } catch (...) {
+        destroy_matrix(c);
+        throw;
+    }
+    return c;
+}
+
 VBH_API int vbh_matrix_duplicate(vbh_matrix_t m, vbh_matrix_t* out, void* stream) {
     return guard([&] {
         VBH_REQUIRE(m && out, "null argument");
-        hipStream_t s = (hipStream_t)stream;
-        vbh_matrix* c = build_matrix(m->nbr, m->nbc, m->h_rbs.data(), m->h_cbs.data(), m->h_row_ptr.data(),
-                                     m->h_col_ind.data(), m->dtype, s);
-        if (m->nnz)
-            VBH_CHECK_HIP(hipMemcpyAsync(c->vals, m->vals, m->elem_size() * m->nnz, hipMemcpyDeviceToDevice, s));
-        *out = c;
+        *out = clone_matrix(m, (hipStream_t)stream);
     });
 }
 
