@@ -300,9 +300,13 @@ __global__ __launch_bounds__(256) void k_numeric(NumArgs<T, CT> a) {
 // still receives its products in A-slot order and, within a product, in
 // inner-index order, exactly as in k_numeric.
 constexpr int kNumMapCols = 8192;  // LDS column map (nbc ints, dynamic) when nbc <= this
+#ifndef VBH_NUM_WAVES
+#define VBH_NUM_WAVES 4
+#endif
+constexpr int kNumWaves = VBH_NUM_WAVES;  // waves per work-group, each owning every kNumWaves-th C slot (4 beats 2 and 8)
 
 template <typename T, typename CT, bool MAP>
-__global__ __launch_bounds__(256) void k_numeric_owned(NumArgs<T, CT> a, int nbc) {
+__global__ __launch_bounds__(64 * kNumWaves) void k_numeric_owned(NumArgs<T, CT> a, int nbc) {
     constexpr bool NS = !std::is_same_v<T, CT>;  // narrow storage: a_c, b_c, c_c
     extern __shared__ int colmap[];  // nbc entries when MAP
     const int i = blockIdx.x;
@@ -311,9 +315,9 @@ __global__ __launch_bounds__(256) void k_numeric_owned(NumArgs<T, CT> a, int nbc
     const int c0 = a.c_row_ptr[i], c1 = a.c_row_ptr[i + 1];
     if (c0 == c1) return;
     if constexpr (MAP) {
-        for (int j = tid; j < nbc; j += 256) colmap[j] = -1;
+        for (int j = tid; j < nbc; j += 64 * kNumWaves) colmap[j] = -1;
         __syncthreads();
-        for (int q = c0 + tid; q < c1; q += 256) colmap[a.c_col_ind[q]] = q;
+        for (int q = c0 + tid; q < c1; q += 64 * kNumWaves) colmap[a.c_col_ind[q]] = q;
         __syncthreads();
     }
     const int r = a.a_rbs[i];
@@ -351,7 +355,7 @@ __global__ __launch_bounds__(256) void k_numeric_owned(NumArgs<T, CT> a, int nbc
                     }
                 }
             }
-            const bool mine = pos >= 0 && ((pos - c0) & 3) == wave;
+            const bool mine = pos >= 0 && (pos - c0) % kNumWaves == wave;
             uint64_t mask = __ballot(mine);
             used += __popcll(mask);
             int64_t boff = 0, coff = 0;
@@ -843,10 +847,11 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
                             VBH_CHECK_HIP(hipMemsetAsync(na.c_c, 0, sizeof(CT) * size_t(C->nnz), s));
                         }
                         if (C->nbc <= kNumMapCols)
-                            hipLaunchKernelGGL((k_numeric_owned<T, CT, true>), dim3(nbr), dim3(256),
+                            hipLaunchKernelGGL((k_numeric_owned<T, CT, true>), dim3(nbr), dim3(64 * kNumWaves),
                                                sizeof(int) * size_t(C->nbc), s, na, C->nbc);
                         else
-                            hipLaunchKernelGGL((k_numeric_owned<T, CT, false>), dim3(nbr), dim3(256), 0, s, na, C->nbc);
+                            hipLaunchKernelGGL((k_numeric_owned<T, CT, false>), dim3(nbr), dim3(64 * kNumWaves), 0, s, na,
+                                               C->nbc);
                         if constexpr (narrow_products) {
                             VBH_CHECK_HIP(hipGetLastError());
                             hipLaunchKernelGGL((k_widen_values<T, CT>), dim3(unsigned((C->nnz + 255) / 256)), dim3(256),
