@@ -236,15 +236,17 @@ VBH_API int vbh_matrix_block_norms(vbh_matrix_t m, double* norms_dev, void* stre
  * norm < thr.  *out receives a new matrix, or NULL when nothing was dropped. */
 VBH_API int vbh_matrix_filter(vbh_matrix_t m, double thr, vbh_matrix_t* out, void* stream);
 /* self <- alpha X + beta self (ops/axpby.hpp:254-528): same pattern in place
- * (*out = NULL) or the union pattern into a new matrix *out. */
+ * (*out = NULL), or the union pattern into a new matrix *out with self left
+ * unchanged. */
 VBH_API int vbh_matrix_axpby(vbh_matrix_t self, const double* alpha, vbh_matrix_t X,
                              const double* beta, vbh_matrix_t* out, void* stream);
 /* Thresholded block SpGEMM C = A B (BlockSpMat::spmm, block_csr.hpp:1530;
  * symbolic ops/spmm/common.hpp:507-735, numeric ops/spmm/vbcsr.hpp:206-471,
  * filter block_csr.hpp:1399).  A: nbr x nbk blocks, B: nbk x nbc; B's block
  * rows must be A's local columns.  numeric: 0 = native FP64/complex128,
- * 1 = FP32 block products, 2 = BF16 block products on MFMA (FP32 accumulate;
- * real matrices only).  Result in *out.  stats (optional, 4 int64): symbolic
+ * 1 = FP32 block products (from FP32 copies of A's and B's values, kept on
+ * the device as for precision-1 applies), 2 = BF16 block products on MFMA
+ * (FP32 accumulate; real matrices only).  Result in *out.  stats (optional, 4 int64): symbolic
  * pairs, numeric pairs, C blocks before filter, after. */
 VBH_API int vbh_spgemm(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric, vbh_matrix_t* out,
                        int64_t* stats, void* stream);
