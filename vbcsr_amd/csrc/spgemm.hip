@@ -296,7 +296,9 @@ __global__ __launch_bounds__(256) void k_numeric(NumArgs<T, CT> a) {
 // LDS map of the row when MAP, else a per-lane binary search), and the
 // offsets, all lane-parallel; the pairs the wave owns are compacted by a
 // ballot and processed one after another by lanes (a, b) = (lane / 8,
-// lane % 8), each holding its row of A_ik in registers.  Every C element
+// lane % 8), each holding its row of A_ik in registers; for real types a B
+// block is loaded whole (one coalesced load, element e in lane e) and each
+// lane gathers its column b by cross-lane shuffles.  Every C element
 // still receives its products in A-slot order and, within a product, in
 // inner-index order, exactly as in k_numeric.
 constexpr int kNumMapCols = 8192;  // LDS column map (nbc ints, dynamic) when nbc <= this
@@ -392,11 +394,33 @@ __global__ __launch_bounds__(64 * kNumWaves) void k_numeric_owned(NumArgs<T, CT>
                     co[u] = int64_t((uint64_t(uint32_t(__builtin_amdgcn_readlane(int(coff >> 32), t))) << 32) |
                                     uint32_t(__builtin_amdgcn_readlane(int(coff), t)));
                     const int cbc = cb < cc[u] ? cb : cc[u] - 1;
-                    if constexpr (NS) {
+                    if constexpr (NS && std::is_same_v<CT, float>) {
+                        acc[u] = a.c_c[co[u] + rac * cc[u] + cbc];
+                        const int nb = m * cc[u];
+                        const float ball = a.b_c[bo[u] + (lane < nb ? lane : nb - 1)];
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) {
+                            const int src = q * cc[u] + cbc;
+                            const float v = __shfl(ball, src < nb ? src : nb - 1, 64);
+                            bv[u][q] = q < m ? v : 0.0f;
+                        }
+                    } else if constexpr (NS) {
                         acc[u] = a.c_c[co[u] + rac * cc[u] + cbc];
 #pragma unroll
                         for (int q = 0; q < 8; ++q)
                             bv[u][q] = q < m ? a.b_c[bo[u] + int64_t(q) * cc[u] + cbc] : Sc<CT>::zero();
+                    } else if constexpr (std::is_same_v<T, double>) {
+                        // the whole B block in one coalesced load (lane e holds
+                        // element e), then each lane takes its column by shuffles
+                        acc[u] = a.c_vals[co[u] + rac * cc[u] + cbc];
+                        const int nb = m * cc[u];
+                        const double ball = a.b_vals[bo[u] + (lane < nb ? lane : nb - 1)];
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) {
+                            const int src = q * cc[u] + cbc;
+                            const double v = __shfl(ball, src < nb ? src : nb - 1, 64);
+                            bv[u][q] = q < m ? v : 0.0;
+                        }
                     } else {
                         acc[u] = narrow<T, CT>(a.c_vals[co[u] + rac * cc[u] + cbc]);
 #pragma unroll
