@@ -248,6 +248,9 @@ void select_row_classes(vbh_matrix* m, int t, hipStream_t s) {
             L.n_small = int64_t(small.size());
             L.big_desc = upload(desc, s);
             L.small_rows = upload(small, s);
+            // the lists may be built on one stream (an apply's first call uses
+            // the null stream) and read by kernels on another
+            VBH_CHECK_HIP(hipStreamSynchronize(s));
         }
         found = m->row_lists.emplace(t, L).first;
     }
