@@ -72,6 +72,18 @@ VBH_API int vbh_memcpy(void* dst, const void* src, size_t bytes, int kind /*0 h2
                        void* stream);
 VBH_API int vbh_memset(void* dst, int value, size_t bytes, void* stream);
 VBH_API int vbh_stream_sync(void* stream);
+/* Threading.  The library keeps per-(device, stream) state: scratch buffers
+ * and cached launch objects (the batched-KPM graph).  Every call holds the
+ * state of its stream for its whole duration, so calls on different streams
+ * may run concurrently from different threads, and calls on one stream are
+ * serialised.  Structures a matrix builds on first use (FP32 value copy,
+ * row-class lists, adjoint plan) are built under the matrix's own lock and
+ * are complete on the device before any kernel reads them.  Writing a
+ * matrix's values while another call reads them is the caller's race, as in
+ * any library.  vbh_stream_release frees the state of a stream the caller is
+ * about to destroy (no reference counterpart: the reference holds the GIL
+ * and runs no streams). */
+VBH_API int vbh_stream_release(void* stream);
 
 /* ---- matrix lifetime and values --------------------------------------- */
 /* Build the device structure from host CSR arrays (DistGraph local view,
@@ -89,8 +101,15 @@ VBH_API int vbh_matrix_get_pattern(vbh_matrix_t m, int* row_ptr, int* col_ind);
  * host pointer, 1 device pointer. */
 VBH_API int vbh_matrix_set_values(vbh_matrix_t m, const void* vals, int location, void* stream);
 VBH_API int vbh_matrix_get_values(vbh_matrix_t m, void* vals, int location, void* stream);
-/* Device pointer to the values (for in-place kernels such as the adjoint). */
+/* Device pointer to the values (for in-place kernels such as the adjoint).
+ * After writing through it, call vbh_matrix_values_changed. */
 VBH_API void* vbh_matrix_values(vbh_matrix_t m);
+/* The values were written through vbh_matrix_values (or by any device copy
+ * the library did not make): copies derived from them (the FP32 copy of
+ * precisions 1 and 2) are rebuilt on next use.  The counterpart of the
+ * reference's in-place value writes (add_block INSERT/ADD after assembly,
+ * block_csr.hpp:613-700), which have no derived copies to refresh. */
+VBH_API int vbh_matrix_values_changed(vbh_matrix_t m);
 /* Deep copy (block_csr.hpp:579 duplicate). */
 VBH_API int vbh_matrix_duplicate(vbh_matrix_t m, vbh_matrix_t* out, void* stream);
 

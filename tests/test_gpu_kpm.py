@@ -178,3 +178,86 @@ def test_cheb_apply_vs_oracle_complex():
     assert relerr(y, ref) < 1e-12
     # e^{-iHt} is unitary: the norm is conserved to the truncation error
     assert abs(np.linalg.norm(y) - 1.0) < 1e-10
+
+
+def test_spectral_bounds_device_lanczos():
+    """Device Lanczos bounds against dense eigenvalues (small H, enough steps
+    to converge) and scipy's eigsh (larger H)."""
+    import scipy.sparse.linalg as sla
+
+    from vbcsr_amd import kpm as K
+    from vbcsr_amd import workloads as W
+
+    Hs = W.batched_hamiltonians(count=2, nb_lo=30, nb_hi=40, neighbours=8.0, seed=21, scale=False)
+    for A in Hs:
+        w = np.linalg.eigvalsh(A.to_dense())
+        lo, hi = K.spectral_bounds(mk(A), num_iter=A.n)
+        assert abs(lo - w[0]) < 1e-8 * np.abs(w).max() and abs(hi - w[-1]) < 1e-8 * np.abs(w).max()
+    A = W.batched_hamiltonians(count=1, nb_lo=600, nb_hi=601, seed=22, scale=False)[0]
+    S = A.to_scipy()
+    e_hi = sla.eigsh(S, k=1, which="LA", tol=1e-10, return_eigenvectors=False)[0]
+    e_lo = sla.eigsh(S, k=1, which="SA", tol=1e-10, return_eigenvectors=False)[0]
+    lo, hi = K.spectral_bounds(mk(A), num_iter=160)
+    width = e_hi - e_lo
+    # Ritz values converge from inside the spectrum
+    assert e_lo - 1e-9 * width <= lo < e_lo + 1e-3 * width
+    assert e_hi - 1e-3 * width < hi <= e_hi + 1e-9 * width
+
+
+def test_batched_c4_dos_bar_spectrum_scaled():
+    """Batched uneven KPM as configured: random-geometric Hamiltonians with
+    blocks {1,4,9,13}, ~20 neighbours, each spectrum scaled to [-0.99, 0.99]
+    by its own Lanczos bounds, 512 moments, 32 Rademacher vectors each.  The
+    FP32-values / FP64-vectors recursion ("mixed") meets the DOS-error bar
+    1e-5 against the plain FP64 path on every Hamiltonian; the all-FP32
+    recursion is recorded against a looser bound (its error is set by the
+    FP32 rounding of the vectors, largest on the smallest Hamiltonians)."""
+    from vbcsr_amd import kpm as K
+    from vbcsr_amd import workloads as W
+
+    Hs = W.batched_hamiltonians(count=8, nb_lo=200, nb_hi=1000, seed=31, scale=False)
+    mats = [mk(A) for A in Hs]
+    for H in mats:
+        K.scale_to_unit(H, 0.99)
+    # the scaled spectra sit at the configured edges
+    for H in mats[:3]:
+        lo, hi = K.spectral_bounds(H)
+        assert -0.991 < lo < -0.985 and 0.985 < hi < 0.991, (lo, hi)
+    V0s = []
+    for h, H in enumerate(mats):
+        v = H.create_multivector(32)
+        v.set_random(1000 + h, normalize=False)
+        V0s.append(v)
+    M = 512
+    mu64 = [K.kpm_moments(H, v, M, mode="plain") for H, v in zip(mats, V0s)]
+    mub = K.kpm_moments_batched(mats, V0s, M, mode="two", precision="mixed")
+    mu32 = K.kpm_moments_batched(mats, V0s, M, mode="two", precision="fp32")
+    worst_mixed = worst_fp32 = 0.0
+    for h in range(len(mats)):
+        _, r64 = K.dos_from_moments(mu64[h])
+        worst_mixed = max(worst_mixed, K.dos_error(K.dos_from_moments(mub[h])[1], r64))
+        worst_fp32 = max(worst_fp32, K.dos_error(K.dos_from_moments(mu32[h])[1], r64))
+    print(f"C4 DOS error vs FP64: mixed {worst_mixed:.2e}, fp32 {worst_fp32:.2e}")
+    assert worst_mixed <= 1e-5
+    assert worst_fp32 <= 2e-4
+
+
+def test_c2_fp32_dos_error_full_lattice():
+    """Graphene 200 x 200 cells, 4 orbitals (N = 320,000), spectrum scaled to
+    [-0.99, 0.99], 200 moments, 10 Rademacher vectors: the FP32 recursion's
+    DOS error against FP64 stays within 1e-5 at the configured size
+    (examples/ex08_graphene_dos.py:145-155 reconstruction, interior nodes)."""
+    from vbcsr_amd import kpm as K
+    from vbcsr_amd import workloads as W
+
+    A = W.graphene(200, 200, 4, seed=0, scale=False)
+    H = mk(A)
+    K.scale_to_unit(H, 0.99)
+    V0 = H.create_multivector(10)
+    V0.set_random(42, normalize=False)
+    mu64 = K.kpm_moments(H, V0, 200, mode="plain")
+    mu32 = K.kpm_moments(H, V0, 200, mode="two", precision="fp32")
+    _, r64 = K.dos_from_moments(mu64)
+    err = K.dos_error(K.dos_from_moments(mu32)[1], r64)
+    print(f"C2 full lattice FP32 DOS error {err:.2e}")
+    assert err <= 1e-5

@@ -142,7 +142,7 @@ extern "C" {
 
 VBH_API int vbh_cheb_basis(vbh_matrix_t m, int K, const void* X, int ldx, int nterms, void* T, int ldt, double center,
                            double half_width, int precision, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(m && X && T, "null argument");
         VBH_REQUIRE(m->nbc == m->nbr, "the Chebyshev basis needs a square local pattern without ghosts");
         VBH_REQUIRE(K >= 1 && nterms >= 1 && ldx >= K && ldt >= int64_t(nterms) * K, "ldt must be >= nterms*K");
@@ -158,7 +158,7 @@ VBH_API int vbh_cheb_basis(vbh_matrix_t m, int K, const void* X, int ldx, int nt
 
 VBH_API int vbh_cheb_adjoint(vbh_matrix_t m, int K, const void* L, int ldl, int nterms, const double* coef, void* B,
                              int ldb, double center, double half_width, int precision, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(m && L && B && coef, "null argument");
         VBH_REQUIRE(m->nbc == m->nbr, "the Chebyshev adjoint needs a square local pattern without ghosts");
         VBH_REQUIRE(K >= 1 && nterms >= 1 && ldl >= K && ldb >= int64_t(nterms) * K, "ldb must be >= nterms*K");
@@ -174,7 +174,7 @@ VBH_API int vbh_cheb_adjoint(vbh_matrix_t m, int K, const void* L, int ldl, int 
 
 VBH_API int vbh_sddmm(vbh_matrix_t m, int L, const void* U, int ldu, const void* V, int ldv, const double* alpha,
                       void* G, int precision, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(m && G && alpha, "null argument");
         VBH_REQUIRE(L >= 0 && ldu >= L && ldv >= L, "leading dimensions must be >= L");
         VBH_REQUIRE(precision == 0 || precision == 1, "precision must be 0 or 1");

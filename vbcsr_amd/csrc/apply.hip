@@ -165,7 +165,7 @@ void launch_rows(const vbh_matrix* m, const ApplyArgs<T, V>& a, hipStream_t s) {
             b.G = gl;
             b.row_list = mf ? 1 : 0;
             b.parts = sh.P;
-            launch_lanes<T, V>(b, mf ? m->n_small : m->n_rows, sh, m->uniform_bs == 4 ? 4 : 0, s);
+            launch_lanes<T, V>(b, mf ? a.p.n_small : m->n_rows, sh, m->uniform_bs == 4 ? 4 : 0, s);
         }
         if (mf) {
             ApplyArgs<T, V> c = a;
@@ -224,12 +224,12 @@ bool launch_spmv_series(const vbh_matrix* m, const ApplyArgs<T, V>& a, const Ser
 
 template <typename T, typename V>
 ApplyArgs<T, V> make_args(const vbh_matrix* m, const V* vals, int K, const T* X, int64_t ldx, T* Y, int64_t ldy) {
-    // row-class lists of the active matrix-core threshold, built on first use
+    // row-class lists of the default matrix-core threshold, built on first use
     // (before any group count is derived from them; never inside a capture:
-    // the captured batched launches select their lists beforehand)
-    select_row_classes(const_cast<vbh_matrix*>(m), m->mfma_min_rows, nullptr);
+    // the captured batched launches take their lists beforehand)
+    const auto& L = row_classes(const_cast<vbh_matrix*>(m), kMfmaMinRows, nullptr);
     ApplyArgs<T, V> a{};
-    a.p = m->pattern();
+    a.p = m->pattern(L);
     a.parts = 1;
     a.vals = vals;
     a.K = K;
@@ -314,7 +314,7 @@ using namespace vbh;
 extern "C" {
 
 VBH_API int vbh_spmm(vbh_matrix_t m, int K, const void* X, int ldx, void* Y, int ldy, int precision, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(m, "null matrix");
         VBH_REQUIRE(K >= 0 && ldx >= K && ldy >= K, "leading dimensions must be >= K");
         hipStream_t s = (hipStream_t)stream;
@@ -332,7 +332,7 @@ VBH_API int vbh_spmv(vbh_matrix_t m, const void* x, void* y, int precision, void
 
 VBH_API int vbh_spmm_adjoint(vbh_matrix_t m, int K, const void* X, int ldx, void* Y, int ldy, int precision,
                              void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(m, "null matrix");
         VBH_REQUIRE(K >= 0 && ldx >= K && ldy >= K, "leading dimensions must be >= K");
         hipStream_t s = (hipStream_t)stream;
@@ -351,7 +351,7 @@ VBH_API int vbh_spmv_adjoint(vbh_matrix_t m, const void* x, void* y, int precisi
 VBH_API int vbh_spmm_fused(vbh_matrix_t m, int K, const void* X, int ldx, void* Y, int ldy, double alpha,
                            double beta, double gamma, const void* Z, int ldz, const void* W, int ldw, double* dots,
                            int precision, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(m, "null matrix");
         VBH_REQUIRE(K >= 0 && ldx >= K && ldy >= K, "leading dimensions must be >= K");
         VBH_REQUIRE(gamma == 0.0 || (Z && ldz >= K), "gamma != 0 needs Z with ldz >= K");

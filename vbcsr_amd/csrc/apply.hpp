@@ -511,19 +511,19 @@ inline int mfma_groups(int n_big) {
 }
 template <typename T, typename V>
 inline bool uses_mfma(const vbh_matrix* m, const ApplyArgs<T, V>& a) {
-    return !Sc<T>::is_complex && m->n_big > 0 && m->max_block <= 16 && uses_lanes(a);
+    return !Sc<T>::is_complex && a.p.n_big > 0 && m->max_block <= 16 && uses_lanes(a);
 }
 // Work-groups of the lane part (over the small rows when the matrix-core
 // class runs apart) and of the matrix-core part.
 template <typename T, typename V>
 inline void split_groups(const vbh_matrix* m, const ApplyArgs<T, V>& a, int& g_lanes, int& g_mfma, int* parts = nullptr) {
     const bool mf = uses_mfma(m, a);
-    const int64_t rows = mf ? m->n_small : m->n_rows;
+    const int64_t rows = mf ? a.p.n_small : m->n_rows;
     const int S = lane_shape_of<T, V>(a.K).S;
     const int P = lane_parts(rows, S, m->uniform_bs == 4 ? 4 : 0, int(sizeof(T)));
     if (parts) *parts = P;
     g_lanes = rows > 0 ? lane_groups(rows, S, P) : 0;
-    g_mfma = mf ? mfma_groups(m->n_big) : 0;
+    g_mfma = mf ? mfma_groups(a.p.n_big) : 0;
 }
 // Work-groups (= partial slots) a launch_rows call with these arguments uses.
 template <typename T, typename V>

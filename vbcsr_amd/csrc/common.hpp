@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <map>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -231,13 +232,8 @@ struct vbh_matrix {
     int64_t* voff = nullptr;
     int2* meta = nullptr;
     int* row_of = nullptr;
-    // active row-class lists (select_row_classes); one set per matrix-core
-    // threshold is kept in row_lists
-    int* small_rows = nullptr;
-    int4* big_desc = nullptr;
-    int64_t n_small = 0;
-    int n_big = 0;
-    int mfma_min_rows = 0;
+    // row-class lists, one set per matrix-core threshold t (row_classes()):
+    // the scalar rows of the lane family and the matrix-core descriptors
     struct RowLists {
         int* small_rows = nullptr;
         int4* big_desc = nullptr;
@@ -257,11 +253,17 @@ struct vbh_matrix {
     int *t_ptr = nullptr, *t_row = nullptr, *t_slot = nullptr, *t_split = nullptr;
     int tG = 0;
     bool t_valid = false;
+    // Guards the structures built on demand (FP32 copy, row-class lists,
+    // adjoint plan): calls on several streams may share one matrix.  Each is
+    // complete on the device (its stream synchronised) before it is marked
+    // valid, so a kernel on any stream may read it.
+    std::mutex lazy_mu;
 
-    vbh::DevPattern pattern() const {
-        return vbh::DevPattern{nbr,  nbc,    uniform_bs, rbs,        cbs,       row_ptr, col_ind, roff,
-                               coff, voff,   meta,       row_of,     small_rows, big_desc, n_small, n_big};
+    vbh::DevPattern pattern(const RowLists& L) const {
+        return vbh::DevPattern{nbr,  nbc,  uniform_bs, rbs,          cbs,        row_ptr,   col_ind, roff,
+                               coff, voff, meta,       row_of,       L.small_rows, L.big_desc, L.n_small, L.n_big};
     }
+    vbh::DevPattern pattern() const { return pattern(RowLists{}); }
     size_t elem_size() const { return dtype == VBH_C128 ? 16 : 8; }
 };
 
@@ -277,14 +279,105 @@ T* upload(const std::vector<T>& h, hipStream_t s = nullptr) {
 }
 // Ensure the FP32 copy of the values exists and is current.
 void ensure_vals32(vbh_matrix* m, hipStream_t s);
+// Mark the device values as changed: the FP32 copy is rebuilt on next use.
+void values_changed(vbh_matrix* m);
 // Ensure the adjoint (column-major incoming) plan exists.
 void ensure_adjoint_plan(vbh_matrix* m, hipStream_t s);
-// Activate the row-class lists of matrix-core threshold t (built on first use).
-void select_row_classes(vbh_matrix* m, int t, hipStream_t s);
+// Row-class lists of matrix-core threshold t (block rows of t..16 rows on the
+// matrix cores), built on first use and kept until the matrix is destroyed.
+const vbh_matrix::RowLists& row_classes(vbh_matrix* m, int t, hipStream_t s);
 // Build a matrix handle from host pattern arrays (no value upload).
 vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const int* row_ptr,
                          const int* col_ind, int dtype, hipStream_t s);
 void destroy_matrix(vbh_matrix* m);
-// Persistent device scratch owned by the library (grown on demand).
+
+// Device buffers of one call, freed when it returns or throws.
+struct DevList {
+    std::vector<void*> bufs;
+    template <typename P>
+    P* own(P* p) {
+        bufs.push_back((void*)p);
+        return p;
+    }
+    DevList() = default;
+    DevList(const DevList&) = delete;
+    DevList& operator=(const DevList&) = delete;
+    ~DevList() {
+        for (void* p : bufs) dfree(p);
+    }
+};
+// A matrix under construction: destroyed unless released to the caller.
+struct MatrixGuard {
+    vbh_matrix* m = nullptr;
+    explicit MatrixGuard(vbh_matrix* x) : m(x) {}
+    MatrixGuard(const MatrixGuard&) = delete;
+    MatrixGuard& operator=(const MatrixGuard&) = delete;
+    ~MatrixGuard() { destroy_matrix(m); }
+    vbh_matrix* release() {
+        vbh_matrix* r = m;
+        m = nullptr;
+        return r;
+    }
+    void reset(vbh_matrix* x) {
+        destroy_matrix(m);
+        m = x;
+    }
+};
+
+// ---------------------------------------------------------------- per-stream state
+// A second stream (default priority) and two events for fork/join of
+// independent launches, eager or under graph capture.
+struct StreamFork {
+    hipStream_t side = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+    void init();
+};
+// A captured batched-KPM graph, the stream it is launched on and the key of
+// the batch it was captured for.
+struct GraphCache {
+    hipStream_t cs = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<uintptr_t> key;
+};
+// Library state of one (device, stream): grow-only scratch slots and cached
+// launch objects.  Every C-ABI call holds the state of its stream for the
+// whole call (StreamScope), so calls on different streams never share a
+// scratch buffer, and calls on one stream run one after another on the host
+// as their work does on the device.  A slot only grows after its stream is
+// synchronised, so no queued kernel still reads the buffer it frees.
+struct StreamState {
+    std::recursive_mutex mu;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<std::pair<void*, size_t>> slots;
+    StreamFork fork;
+    GraphCache graph;
+};
+// RAII: locks the state of (current device, s) and makes it the current
+// state of this thread (scratch() uses the innermost scope).
+class StreamScope {
+public:
+    explicit StreamScope(hipStream_t s);
+    ~StreamScope();
+    StreamScope(const StreamScope&) = delete;
+    StreamScope& operator=(const StreamScope&) = delete;
+    StreamState& state() { return *st_; }
+
+private:
+    StreamState* st_;
+    StreamState* prev_;
+};
+// Scratch slot of the current StreamScope (grown on demand).
 void* scratch(size_t bytes, int slot);
+// State of the current StreamScope.
+StreamState& current_stream_state();
+// guard() with the stream's state held for the call.
+template <typename F>
+int guard_on(void* stream, F&& f) {
+    return guard([&] {
+        StreamScope scope((hipStream_t)stream);
+        f();
+    });
+}
 }  // namespace vbh

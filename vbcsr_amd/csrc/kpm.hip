@@ -134,11 +134,11 @@ __global__ __launch_bounds__(kThreads) void k_apply_batched(const BatchItem<T>* 
     write_partials(1, lane, wave, d1, d2, partials + int64_t(gg) * 2);
 }
 
-// HIP-event timing of the last recursion: the step kernels only (setup,
-// the final reduction and the host copy excluded), recorded on the launch
-// stream.  Read through vbh_kpm_last_step_time.
-static double g_last_step_ms = 0.0;
-static int64_t g_last_launches = 0;
+// HIP-event timing of the last recursion on this thread: the step kernels
+// only (setup, the final reduction and the host copy excluded), recorded on
+// the launch stream.  Read through vbh_kpm_last_step_time.
+static thread_local double g_last_step_ms = 0.0;
+static thread_local int64_t g_last_launches = 0;
 
 struct StepTimer {
     hipEvent_t t0 = nullptr, t1 = nullptr;
@@ -159,19 +159,6 @@ struct StepTimer {
     ~StepTimer() {
         if (t0) (void)hipEventDestroy(t0);
         if (t1) (void)hipEventDestroy(t1);
-    }
-};
-
-// A second stream (default priority) and two events for fork/join of
-// independent launches, eager or under graph capture.
-struct StreamFork {
-    hipStream_t side = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
-    void init() {
-        if (side) return;
-        VBH_CHECK_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
-        VBH_CHECK_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
-        VBH_CHECK_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
     }
 };
 
@@ -335,16 +322,17 @@ struct BatchLayout {
     std::vector<int> seg, seg2;
     int Gtot1 = 0, Gtot2 = 0;
 };
-static BatchLayout batch_layout(int count, vbh_matrix* const* ms, const LaneShape& sh, bool mfma_ok) {
+static BatchLayout batch_layout(int count, vbh_matrix* const* ms, const vbh_matrix::RowLists* const* lists,
+                                const LaneShape& sh, bool mfma_ok) {
     BatchLayout L;
     L.seg.assign(count + 1, 0);
     L.seg2.assign(count + 1, 0);
     for (int h = 0; h < count; ++h) {
         const int64_t rpg = sh.S ? lane_rows_per_group(sh.S, sh.P) : 1;
-        const bool split = mfma_ok && ms[h]->n_big > 0;
-        const int64_t lrows = split ? ms[h]->n_small : ms[h]->n_rows;
+        const bool split = mfma_ok && lists[h]->n_big > 0;
+        const int64_t lrows = split ? lists[h]->n_small : ms[h]->n_rows;
         L.seg[h + 1] = L.seg[h] + (sh.S ? int((lrows + rpg - 1) / rpg) : ms[h]->G);
-        L.seg2[h + 1] = L.seg2[h] + (split ? (ms[h]->n_big + kMfmaRowsPerGroup - 1) / kMfmaRowsPerGroup : 0);
+        L.seg2[h + 1] = L.seg2[h] + (split ? (lists[h]->n_big + kMfmaRowsPerGroup - 1) / kMfmaRowsPerGroup : 0);
     }
     L.Gtot1 = (L.seg[count] + 7) & ~7;
     L.Gtot2 = (L.seg2[count] + 7) & ~7;
@@ -360,10 +348,13 @@ struct BatchPlan {
     int P = 1;
     bool tuned = false;
 };
+// Plans per batch (process-wide, guarded by g_plan_mu), the switches, and the
+// plan of this thread's last batched call.
+static std::mutex g_plan_mu;
 static std::map<std::vector<uintptr_t>, BatchPlan> g_batch_plans;
 static int g_autotune = 1;
-static BatchPlan g_last_plan;
 static BatchPlan g_forced_plan{0, 1, false};  // t > 0: used instead of the cached or tuned plan
+static thread_local BatchPlan g_last_plan;
 
 template <typename T, typename S, typename V>
 static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* const* V0s, const int* ldv, int M, int mode,
@@ -382,15 +373,15 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
     LaneShape sh = lane_shape_of<T, V>(K);
     bool mfma_ok = !Sc<T>::is_complex && sh.S > 0;
     for (int h = 0; h < count; ++h) mfma_ok = mfma_ok && ms[h]->max_block <= 16;
-    std::vector<int> t_before(count);
-    for (int h = 0; h < count; ++h) t_before[h] = ms[h]->mfma_min_rows;
+    // row-class lists of every matrix for the plan's matrix-core threshold
+    std::vector<const vbh_matrix::RowLists*> lists(count);
     auto select_all = [&](int t) {
-        for (int h = 0; h < count; ++h) select_row_classes(ms[h], t, s);
+        for (int h = 0; h < count; ++h) lists[h] = &row_classes(ms[h], t, s);
     };
     auto default_parts = [&]() {
         int64_t lrows = 0;
         for (int h = 0; h < count; ++h)
-            lrows += (mfma_ok && ms[h]->n_big > 0) ? ms[h]->n_small : ms[h]->n_rows;
+            lrows += (mfma_ok && lists[h]->n_big > 0) ? lists[h]->n_small : ms[h]->n_rows;
         return lane_parts(lrows, sh.S, 0, int(sizeof(T)));
     };
     std::vector<uintptr_t> pkey = {uintptr_t(count), uintptr_t(K), uintptr_t(sizeof(T)), uintptr_t(sizeof(V)),
@@ -399,22 +390,32 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
         pkey.push_back(uintptr_t(ms[h]));
         pkey.push_back(uintptr_t(ms[h]->nnzb));
     }
-    BatchPlan plan;
-    auto known = g_batch_plans.find(pkey);
-    const bool forced = g_forced_plan.t > 0 && sh.S > 0;
-    const bool tune = !forced && sh.S > 0 && g_autotune && nsteps >= 2 && known == g_batch_plans.end();
+    BatchPlan plan, forced_plan;
+    bool have_known = false;
+    int autotune = 1;
+    {
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        auto known = g_batch_plans.find(pkey);
+        if (known != g_batch_plans.end()) {
+            have_known = true;
+            plan = known->second;
+        }
+        forced_plan = g_forced_plan;
+        autotune = g_autotune;
+    }
+    const bool forced = forced_plan.t > 0 && sh.S > 0;
+    const bool tune = !forced && sh.S > 0 && autotune && nsteps >= 2 && !have_known;
     if (forced) {
-        plan = g_forced_plan;
+        plan = forced_plan;
         plan.P = std::min(plan.P, 64 / sh.S);
-    } else if (known != g_batch_plans.end()) {
-        plan = known->second;
+    } else if (have_known) {
     } else if (sh.S) {
         select_all(kMfmaMinRows);
         plan.P = default_parts();
     }
     select_all(plan.t);
     sh.P = sh.S ? plan.P : 1;
-    BatchLayout lay = batch_layout(count, ms, sh, mfma_ok);
+    BatchLayout lay = batch_layout(count, ms, lists.data(), sh, mfma_ok);
     std::vector<int>& seg = lay.seg;
     std::vector<int>& seg2 = lay.seg2;
     int Gtot1 = lay.Gtot1, Gtot2 = lay.Gtot2;
@@ -427,12 +428,12 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
         for (int h = 0; h < count; ++h) {
             vbh_matrix* m = ms[h];
             Item& it = items[h];
-            it.p = m->pattern();
+            it.p = m->pattern(*lists[h]);
             it.p.bs = 0;
             it.split = m->split;
             it.n_rows = m->n_rows;
-            it.row_list = (mfma_ok && m->n_big > 0) ? 1 : 0;
-            it.lane_rows = it.row_list ? m->n_small : m->n_rows;
+            it.row_list = (mfma_ok && lists[h]->n_big > 0) ? 1 : 0;
+            it.lane_rows = it.row_list ? lists[h]->n_small : m->n_rows;
             it.G = L.seg[h + 1] - L.seg[h];
             it.g_base = L.seg[h];
             it.g_base2 = L.seg2[h];
@@ -463,7 +464,8 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
     }
     const double a1 = 1.0 / half, b1 = -center / half;
     const double a2 = 2.0 / half, b2 = -2.0 * center / half;
-    static StreamFork fork;
+    StreamState& sst = current_stream_state();
+    StreamFork& fork = sst.fork;
     if (sh.S && mfma_ok) fork.init();
     // One recursion step of the lane family (and the forked matrix-core
     // launch) for the items at ditems in layout L.
@@ -509,7 +511,7 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
                 if (P < 1 || P > pmax) continue;
                 LaneShape shc = sh;
                 shc.P = P;
-                BatchLayout Lc = batch_layout(count, ms, shc, mfma_ok);
+                BatchLayout Lc = batch_layout(count, ms, lists.data(), shc, mfma_ok);
                 fill_items(Lc);
                 Item* titems = (Item*)scratch(sizeof(Item) * count, 8);
                 double* tpart = (double*)scratch(sizeof(double) * std::max(Lc.Gtot1 + Lc.Gtot2, 8) * E, 9);
@@ -531,14 +533,18 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
-        g_batch_plans[pkey] = plan;
+        {
+            std::lock_guard<std::mutex> lk(g_plan_mu);
+            g_batch_plans[pkey] = plan;
+        }
         select_all(plan.t);
         sh.P = plan.P;
-        lay = batch_layout(count, ms, sh, mfma_ok);
+        lay = batch_layout(count, ms, lists.data(), sh, mfma_ok);
         Gtot1 = lay.Gtot1;
         Gtot2 = lay.Gtot2;
         Gtot = Gtot1 + Gtot2;
-    } else if (!forced && known == g_batch_plans.end()) {
+    } else if (!forced && !have_known) {
+        std::lock_guard<std::mutex> lk(g_plan_mu);
         g_batch_plans[pkey] = plan;
     }
     g_last_plan = plan;
@@ -581,12 +587,8 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
         if (use_graph) {
             // The captured graph is kept and replayed while the batch (matrices,
             // buffers, shape and scaling) is unchanged; a new batch re-captures.
-            static struct {
-                hipStream_t cs = nullptr;
-                hipGraph_t graph = nullptr;
-                hipGraphExec_t exec = nullptr;
-                std::vector<uintptr_t> key;
-            } cache;
+            // one cached graph per stream state
+            GraphCache& cache = sst.graph;
             std::vector<uintptr_t> key = {uintptr_t(ditems), uintptr_t(vecs), uintptr_t(partials), uintptr_t(count),
                                           uintptr_t(K), uintptr_t(nsteps), uintptr_t(mode), uintptr_t(sizeof(T)),
                                           uintptr_t(Gtot), uintptr_t(Gtot1), uintptr_t(sh.P)};
@@ -642,9 +644,8 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
             if (M >= 1) moments[int64_t(h) * M] = mu0[h];
         }
     }
-    // the matrices keep their own row classes for later single-matrix calls
-    // (the captured graph holds the plan's lists, which stay allocated)
-    for (int h = 0; h < count; ++h) select_row_classes(ms[h], t_before[h], s);
+    // (the captured graph holds the plan's lists, which stay allocated until
+    // the matrices are destroyed)
 }
 
 // ------------------------------------------------------------ Chebyshev apply
@@ -728,7 +729,7 @@ extern "C" {
 
 VBH_API int vbh_kpm_moments(vbh_matrix_t m, int K, const void* V0, int ldv, int M, int mode, double center,
                             double half_width, int precision, double* moments, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(m && V0 && moments, "null argument");
         VBH_REQUIRE(m->nbc == m->nbr, "KPM needs a square local pattern without ghosts");
         VBH_REQUIRE(mode >= 0 && mode <= 2, "mode must be 0, 1 or 2");
@@ -747,7 +748,7 @@ VBH_API int vbh_kpm_moments(vbh_matrix_t m, int K, const void* V0, int ldv, int 
 VBH_API int vbh_kpm_moments_batched(int count, const vbh_matrix_t* ms, int K, const void* const* V0s, const int* ldv,
                                     int M, int mode, double center, double half_width, int precision, int graph,
                                     double* moments, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(count >= 1 && ms && V0s && ldv && moments, "null argument");
         VBH_REQUIRE(half_width > 0.0, "half_width must be positive");
         for (int h = 0; h < count; ++h) {
@@ -765,7 +766,10 @@ VBH_API int vbh_kpm_moments_batched(int count, const vbh_matrix_t* ms, int K, co
 }
 
 VBH_API int vbh_set_autotune(int on) {
-    return guard([&] { g_autotune = on ? 1 : 0; });
+    return guard([&] {
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        g_autotune = on ? 1 : 0;
+    });
 }
 
 VBH_API int vbh_set_batch_plan(int threshold, int parts) {
@@ -773,6 +777,7 @@ VBH_API int vbh_set_batch_plan(int threshold, int parts) {
         VBH_REQUIRE(threshold == 0 || (threshold >= 1 && threshold <= 17 && parts >= 1 && parts <= 64 &&
                                        (parts & (parts - 1)) == 0),
                     "threshold must be 0 (release) or 1..17, with parts a power of two in 1..64");
+        std::lock_guard<std::mutex> lk(g_plan_mu);
         g_forced_plan.t = threshold;
         g_forced_plan.P = parts;
         g_forced_plan.tuned = false;
@@ -798,7 +803,7 @@ VBH_API int vbh_kpm_last_step_time(double* ms, int64_t* launches) {
 
 VBH_API int vbh_cheb_apply(vbh_matrix_t m, int K, const void* X, int ldx, void* Y, int ldy, int nterms,
                            const double* coef, double center, double half_width, int precision, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(m && X && Y && coef, "null argument");
         VBH_REQUIRE(m->nbc == m->nbr, "Chebyshev apply needs a square local pattern without ghosts");
         VBH_REQUIRE(ldx >= K && ldy >= K && K >= 1, "bad leading dimensions");

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <numeric>
 
@@ -39,22 +40,78 @@ void dfree(void* p) {
     if (p) (void)hipFree(p);
 }
 
-// Library scratch: a few grow-only device buffers reused across calls on the
-// current device.  Calls that share a slot must not overlap on one device.
-static std::mutex g_scratch_mu;
-static std::vector<std::pair<void*, size_t>> g_scratch(12, {nullptr, 0});
-void* scratch(size_t bytes, int slot) {
-    std::lock_guard<std::mutex> lk(g_scratch_mu);
-    auto& s = g_scratch.at(slot);
-    if (s.second < bytes) {
-        if (s.first) {
-            VBH_CHECK_HIP(hipDeviceSynchronize());
-            dfree(s.first);
-        }
-        s.first = dalloc(bytes);
-        s.second = bytes;
+// ---------------------------------------------------------------- stream state
+// One StreamState per (device, stream), created on first use and kept until
+// vbh_stream_release or the end of the process.
+static std::mutex g_states_mu;
+static std::map<std::pair<int, uintptr_t>, std::unique_ptr<StreamState>> g_states;
+static thread_local StreamState* t_state = nullptr;
+constexpr int kScratchSlots = 12;
+
+static StreamState* state_of(hipStream_t s) {
+    int dev = 0;
+    VBH_CHECK_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_states_mu);
+    auto& st = g_states[{dev, reinterpret_cast<uintptr_t>(s)}];
+    if (!st) {
+        st.reset(new StreamState());
+        st->device = dev;
+        st->stream = s;
+        st->slots.assign(kScratchSlots, {nullptr, 0});
     }
-    return s.first;
+    return st.get();
+}
+
+StreamScope::StreamScope(hipStream_t s) : st_(state_of(s)), prev_(t_state) {
+    st_->mu.lock();
+    t_state = st_;
+}
+StreamScope::~StreamScope() {
+    t_state = prev_;
+    st_->mu.unlock();
+}
+
+void* scratch(size_t bytes, int slot) {
+    StreamState* st = t_state;
+    if (!st) throw Error(VBH_ERR_STATE, "internal: scratch used outside a stream scope");
+    auto& sl = st->slots.at(slot);
+    if (sl.second < bytes) {
+        if (sl.first) {
+            // kernels queued on this stream may still read the old buffer
+            VBH_CHECK_HIP(hipStreamSynchronize(st->stream));
+            dfree(sl.first);
+            sl = {nullptr, 0};
+        }
+        sl.first = dalloc(bytes);
+        sl.second = bytes;
+    }
+    return sl.first;
+}
+
+StreamState& current_stream_state() {
+    if (!t_state) throw Error(VBH_ERR_STATE, "internal: no stream scope");
+    return *t_state;
+}
+
+void StreamFork::init() {
+    if (side) return;
+    VBH_CHECK_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
+    VBH_CHECK_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    VBH_CHECK_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+}
+
+// Frees everything a stream state holds (its stream synchronised first).
+static void release_state(StreamState* st) {
+    (void)hipStreamSynchronize(st->stream);
+    if (st->graph.cs) (void)hipStreamSynchronize(st->graph.cs);
+    if (st->fork.side) (void)hipStreamSynchronize(st->fork.side);
+    for (auto& sl : st->slots) dfree(sl.first);
+    if (st->graph.exec) (void)hipGraphExecDestroy(st->graph.exec);
+    if (st->graph.graph) (void)hipGraphDestroy(st->graph.graph);
+    if (st->graph.cs) (void)hipStreamDestroy(st->graph.cs);
+    if (st->fork.side) (void)hipStreamDestroy(st->fork.side);
+    if (st->fork.fork) (void)hipEventDestroy(st->fork.fork);
+    if (st->fork.join) (void)hipEventDestroy(st->fork.join);
 }
 
 // ---------------------------------------------------------------- kernels
@@ -219,47 +276,46 @@ static std::vector<int> bfs_order(int nbr, const int* row_ptr, const int* col_in
 // one XCD are then graph neighbours, so the X rows they gather overlap and
 // stay in that XCD's L2.  Each row is computed exactly as in index order; only
 // the order of the rows changes.
-void select_row_classes(vbh_matrix* m, int t, hipStream_t s) {
+const vbh_matrix::RowLists& row_classes(vbh_matrix* m, int t, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(m->lazy_mu);
     auto found = m->row_lists.find(t);
-    if (found == m->row_lists.end()) {
-        const int nbr = m->nbr;
-        const std::vector<int>& rbs = m->h_rbs;
-        auto is_big = [&](int i) { return m->uniform_bs != 4 && rbs[i] >= t && rbs[i] <= 16; };
-        bool any_big = false;
-        for (int i = 0; i < nbr && !any_big; ++i) any_big = is_big(i);
-        vbh_matrix::RowLists L;
-        if (any_big) {
-            if (m->h_bfs.empty()) m->h_bfs = bfs_order(nbr, m->h_row_ptr.data(), m->h_col_ind.data());
-            std::vector<int> small, big;
-            for (int i : m->h_bfs) {
-                if (is_big(i)) big.push_back(i);
-                else
-                    for (int r = m->h_roff[i]; r < m->h_roff[i + 1]; ++r) small.push_back(r);
-            }
-            std::vector<int4> desc(2 * big.size());
-            for (size_t q = 0; q < big.size(); ++q) {
-                const int i = big[q];
-                const int kb = m->h_row_ptr[i];
-                const uint64_t v = uint64_t(m->h_voff[kb]);
-                desc[2 * q] = make_int4(kb, m->h_row_ptr[i + 1], rbs[i], m->h_roff[i]);
-                desc[2 * q + 1] = make_int4(m->h_coff[i], int(uint32_t(v)), int(uint32_t(v >> 32)), i);
-            }
-            L.n_big = int(big.size());
-            L.n_small = int64_t(small.size());
-            L.big_desc = upload(desc, s);
-            L.small_rows = upload(small, s);
-            // the lists may be built on one stream (an apply's first call uses
-            // the null stream) and read by kernels on another
-            VBH_CHECK_HIP(hipStreamSynchronize(s));
+    if (found != m->row_lists.end()) return found->second;
+    const int nbr = m->nbr;
+    const std::vector<int>& rbs = m->h_rbs;
+    auto is_big = [&](int i) { return m->uniform_bs != 4 && rbs[i] >= t && rbs[i] <= 16; };
+    bool any_big = false;
+    for (int i = 0; i < nbr && !any_big; ++i) any_big = is_big(i);
+    vbh_matrix::RowLists L;
+    if (any_big) {
+        if (m->h_bfs.empty()) m->h_bfs = bfs_order(nbr, m->h_row_ptr.data(), m->h_col_ind.data());
+        std::vector<int> small, big;
+        for (int i : m->h_bfs) {
+            if (is_big(i)) big.push_back(i);
+            else
+                for (int r = m->h_roff[i]; r < m->h_roff[i + 1]; ++r) small.push_back(r);
         }
-        found = m->row_lists.emplace(t, L).first;
+        std::vector<int4> desc(2 * big.size());
+        for (size_t q = 0; q < big.size(); ++q) {
+            const int i = big[q];
+            const int kb = m->h_row_ptr[i];
+            const uint64_t v = uint64_t(m->h_voff[kb]);
+            desc[2 * q] = make_int4(kb, m->h_row_ptr[i + 1], rbs[i], m->h_roff[i]);
+            desc[2 * q + 1] = make_int4(m->h_coff[i], int(uint32_t(v)), int(uint32_t(v >> 32)), i);
+        }
+        L.n_big = int(big.size());
+        L.n_small = int64_t(small.size());
+        L.big_desc = upload(desc, s);
+        try {
+            L.small_rows = upload(small, s);
+            // read by kernels on any stream: complete before it is published
+            VBH_CHECK_HIP(hipStreamSynchronize(s));
+        } catch (...) {
+            dfree(L.big_desc);
+            dfree(L.small_rows);
+            throw;
+        }
     }
-    const vbh_matrix::RowLists& L = found->second;
-    m->mfma_min_rows = t;
-    m->small_rows = L.small_rows;
-    m->big_desc = L.big_desc;
-    m->n_small = L.n_small;
-    m->n_big = L.n_big;
+    return m->row_lists.emplace(t, L).first->second;
 }
 
 vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const int* row_ptr, const int* col_ind,
@@ -361,7 +417,6 @@ vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const
         // the row-class lists (breadth-first order, matrix-core descriptors) are
         // built by the first apply (make_args): products, filtered copies and
         // sums that are never applied to vectors do not pay for them
-        m->mfma_min_rows = kMfmaMinRows;
         m->split = upload(split, s);
         // kVecPad trailing elements: the streaming kernels read whole 16-byte
         // words, which may reach up to one word past the last value
@@ -390,6 +445,7 @@ void destroy_matrix(vbh_matrix* m) {
 }
 
 void ensure_vals32(vbh_matrix* m, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(m->lazy_mu);
     if (m->vals32_valid) return;
     if (!m->vals32) {
         m->vals32 = dalloc((m->elem_size() / 2) * (m->nnz + kVecPad));
@@ -405,13 +461,21 @@ void ensure_vals32(vbh_matrix* m, hipStream_t s) {
                                (const c128*)m->vals, (c64*)m->vals32, n);
         VBH_CHECK_HIP(hipGetLastError());
     }
+    // read by kernels on any stream: complete before it is marked valid
+    VBH_CHECK_HIP(hipStreamSynchronize(s));
     m->vals32_valid = true;
+}
+
+void values_changed(vbh_matrix* m) {
+    std::lock_guard<std::mutex> lk(m->lazy_mu);
+    m->vals32_valid = false;
 }
 
 // Column-major "incoming" plan for the adjoint: for every local column j the
 // (row, slot) pairs in ascending row order -- one writer per output block, as
 // in the reference's adjoint plan (vbcsr_backend.hpp:543-625).
 void ensure_adjoint_plan(vbh_matrix* m, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(m->lazy_mu);
     if (m->t_valid) return;
     std::vector<int> tp(m->nbc + 1, 0), tr(m->nnzb), ts(m->nnzb);
     for (int64_t k = 0; k < m->nnzb; ++k) tp[m->h_col_ind[k] + 1]++;
@@ -500,6 +564,29 @@ VBH_API int vbh_memset(void* dst, int value, size_t bytes, void* stream) {
 VBH_API int vbh_stream_sync(void* stream) {
     return guard([&] { VBH_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream)); });
 }
+VBH_API int vbh_stream_release(void* stream) {
+    return guard([&] {
+        int dev = 0;
+        VBH_CHECK_HIP(hipGetDevice(&dev));
+        std::unique_ptr<StreamState> st;
+        {
+            std::lock_guard<std::mutex> lk(g_states_mu);
+            auto it = g_states.find({dev, reinterpret_cast<uintptr_t>(stream)});
+            if (it == g_states.end()) return;
+            VBH_REQUIRE(t_state != it->second.get(), "vbh_stream_release inside a call on that stream");
+            std::lock_guard<std::recursive_mutex> busy(it->second->mu);  // no call in flight on it
+            st = std::move(it->second);
+            g_states.erase(it);
+        }
+        release_state(st.get());
+    });
+}
+VBH_API int vbh_matrix_values_changed(vbh_matrix_t m) {
+    return guard([&] {
+        VBH_REQUIRE(m, "null matrix");
+        values_changed(m);
+    });
+}
 
 VBH_API int vbh_matrix_create(int nbr, int nbc, const int* rbs, const int* cbs, const int* row_ptr,
                               const int* col_ind, int dtype, vbh_matrix_t* out) {
@@ -538,7 +625,7 @@ VBH_API int vbh_matrix_get_pattern(vbh_matrix_t m, int* row_ptr, int* col_ind) {
 }
 
 VBH_API int vbh_matrix_set_values(vbh_matrix_t m, const void* vals, int location, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(m, "null matrix");
         const size_t bytes = m->elem_size() * m->nnz;
         if (bytes)
@@ -546,12 +633,12 @@ VBH_API int vbh_matrix_set_values(vbh_matrix_t m, const void* vals, int location
                                          location ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
                                          (hipStream_t)stream));
         if (!location) VBH_CHECK_HIP(hipStreamSynchronize((hipStream_t)stream));
-        m->vals32_valid = false;
+        values_changed(m);
     });
 }
 
 VBH_API int vbh_matrix_get_values(vbh_matrix_t m, void* vals, int location, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(m, "null matrix");
         const size_t bytes = m->elem_size() * m->nnz;
         if (bytes)
@@ -589,7 +676,6 @@ static vbh_matrix* clone_matrix(const vbh_matrix* m, hipStream_t s) {
         c->h_roff = m->h_roff;
         c->h_coff = m->h_coff;
         c->h_bfs = m->h_bfs;
-        c->mfma_min_rows = m->mfma_min_rows;
         c->G = m->G;
         auto copy = [&](auto* src, int64_t count) {
             using E = std::remove_pointer_t<decltype(src)>;
@@ -619,14 +705,14 @@ static vbh_matrix* clone_matrix(const vbh_matrix* m, hipStream_t s) {
 }
 
 VBH_API int vbh_matrix_duplicate(vbh_matrix_t m, vbh_matrix_t* out, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(m && out, "null argument");
         *out = clone_matrix(m, (hipStream_t)stream);
     });
 }
 
 VBH_API int vbh_matrix_scale(vbh_matrix_t m, const double* alpha, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(m && alpha, "null argument");
         hipStream_t s = (hipStream_t)stream;
         if (m->nnz == 0) return;
@@ -640,17 +726,18 @@ VBH_API int vbh_matrix_scale(vbh_matrix_t m, const double* alpha, void* stream) 
                                    m->nnz, scalar_from<T>(alpha));
         });
         VBH_CHECK_HIP(hipGetLastError());
-        m->vals32_valid = false;
+        values_changed(m);
     });
 }
 
 VBH_API int vbh_matrix_shift(vbh_matrix_t m, const double* alpha, const int* diag_col_host, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(m && alpha, "null argument");
         hipStream_t s = (hipStream_t)stream;
         if (m->nbr == 0) return;
         auto ds = diag_slots(m, diag_col_host);
-        int* d = upload(ds, s);
+        DevList tmp;
+        int* d = tmp.own(upload(ds, s));
         dispatch_dtype(m->dtype, [&](auto tag) {
             using T = decltype(tag);
             hipLaunchKernelGGL(k_shift<T>, dim3(blocks_for(m->nbr, 256)), dim3(256), 0, s, (T*)m->vals, m->voff, d,
@@ -658,13 +745,12 @@ VBH_API int vbh_matrix_shift(vbh_matrix_t m, const double* alpha, const int* dia
         });
         VBH_CHECK_HIP(hipGetLastError());
         VBH_CHECK_HIP(hipStreamSynchronize(s));
-        dfree(d);
-        m->vals32_valid = false;
+        values_changed(m);
     });
 }
 
 VBH_API int vbh_matrix_block_norms(vbh_matrix_t m, double* norms_dev, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(m && norms_dev, "null argument");
         if (m->nnzb == 0) return;
         hipStream_t s = (hipStream_t)stream;
@@ -680,9 +766,11 @@ VBH_API int vbh_matrix_block_norms(vbh_matrix_t m, double* norms_dev, void* stre
 // New matrix with the given pattern whose blocks are gathered from m's slots.
 static vbh_matrix* gather_matrix(vbh_matrix* m, const std::vector<int>& rp, const std::vector<int>& ci,
                                  const std::vector<int>& src_slot, hipStream_t s) {
-    vbh_matrix* c = build_matrix(m->nbr, m->nbc, m->h_rbs.data(), m->h_cbs.data(), rp.data(), ci.data(), m->dtype, s);
+    MatrixGuard g(build_matrix(m->nbr, m->nbc, m->h_rbs.data(), m->h_cbs.data(), rp.data(), ci.data(), m->dtype, s));
+    vbh_matrix* c = g.m;
     if (c->nnzb) {
-        int* d_src = upload(src_slot, s);
+        DevList tmp;
+        int* d_src = tmp.own(upload(src_slot, s));
         dispatch_dtype(m->dtype, [&](auto tag) {
             using T = decltype(tag);
             hipLaunchKernelGGL(k_gather_blocks<T>, dim3(blocks_for(c->nnzb, 4)), dim3(256), 0, s,
@@ -690,13 +778,12 @@ static vbh_matrix* gather_matrix(vbh_matrix* m, const std::vector<int>& rp, cons
         });
         VBH_CHECK_HIP(hipGetLastError());
         VBH_CHECK_HIP(hipStreamSynchronize(s));
-        dfree(d_src);
     }
-    return c;
+    return g.release();
 }
 
 VBH_API int vbh_matrix_filter(vbh_matrix_t m, double thr, vbh_matrix_t* out, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(m && out, "null argument");
         *out = nullptr;
         if (!(thr > 0.0) || m->nnzb == 0) return;  // block_csr.hpp:1400
@@ -729,7 +816,7 @@ VBH_API int vbh_matrix_filter(vbh_matrix_t m, double thr, vbh_matrix_t* out, voi
 
 VBH_API int vbh_matrix_axpby(vbh_matrix_t self, const double* alpha, vbh_matrix_t X, const double* beta,
                              vbh_matrix_t* out, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(self && X && alpha && beta && out, "null argument");
         VBH_REQUIRE(self->dtype == X->dtype, "axpby: dtype mismatch");
         VBH_REQUIRE(self->nbr == X->nbr, "Matrix row count mismatch in axpby");
@@ -750,7 +837,7 @@ VBH_API int vbh_matrix_axpby(vbh_matrix_t self, const double* alpha, vbh_matrix_
                                        scalar_from<T>(beta));
             });
             VBH_CHECK_HIP(hipGetLastError());
-            self->vals32_valid = false;
+            values_changed(self);
             return;
         }
         VBH_REQUIRE(self->h_rbs == X->h_rbs, "axpby: block row sizes differ");
@@ -777,11 +864,13 @@ VBH_API int vbh_matrix_axpby(vbh_matrix_t self, const double* alpha, vbh_matrix_
             }
             rp[i + 1] = int(ci.size());
         }
-        vbh_matrix* c = build_matrix(self->nbr, nbc, self->h_rbs.data(), cbs.data(), rp.data(), ci.data(),
-                                     self->dtype, s);
+        MatrixGuard g(build_matrix(self->nbr, nbc, self->h_rbs.data(), cbs.data(), rp.data(), ci.data(),
+                                   self->dtype, s));
+        vbh_matrix* c = g.m;
         if (c->nnzb) {
-            int* d_ys = upload(ys, s);
-            int* d_xs = upload(xs, s);
+            DevList tmp;
+            int* d_ys = tmp.own(upload(ys, s));
+            int* d_xs = tmp.own(upload(xs, s));
             dispatch_dtype(self->dtype, [&](auto tag) {
                 using T = decltype(tag);
                 hipLaunchKernelGGL(k_union_axpby<T>, dim3(blocks_for(c->nnzb, 4)), dim3(256), 0, s,
@@ -790,21 +879,20 @@ VBH_API int vbh_matrix_axpby(vbh_matrix_t self, const double* alpha, vbh_matrix_
             });
             VBH_CHECK_HIP(hipGetLastError());
             VBH_CHECK_HIP(hipStreamSynchronize(s));
-            dfree(d_ys);
-            dfree(d_xs);
         }
-        *out = c;
+        *out = g.release();
     });
 }
 
 VBH_API int vbh_matrix_trace(vbh_matrix_t m, double* out, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(m && out, "null argument");
         hipStream_t s = (hipStream_t)stream;
         out[0] = 0.0;
         if (m->nbr == 0) return;
         auto ds = diag_slots(m, nullptr);
-        int* d = upload(ds, s);
+        DevList tmp;
+        int* d = tmp.own(upload(ds, s));
         const int nb = 64;
         double* part = (double*)scratch(sizeof(double) * nb, 1);
         dispatch_dtype(m->dtype, [&](auto tag) {
@@ -816,7 +904,6 @@ VBH_API int vbh_matrix_trace(vbh_matrix_t m, double* out, void* stream) {
         std::vector<double> h(nb);
         VBH_CHECK_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, s));
         VBH_CHECK_HIP(hipStreamSynchronize(s));
-        dfree(d);
         double t = 0.0;
         for (double v : h) t += v;
         out[0] = t;

@@ -426,7 +426,12 @@ struct Mat {
             fresh = Fresh::Both;
         }
     }
-    void device_written() { fresh = Fresh::Device; }
+    // values written on the device (by a kernel or a device copy): the host
+    // copy is stale, and so is any FP32 copy the library derived from them
+    void device_written() {
+        fresh = Fresh::Device;
+        if (h) check(vbh_matrix_values_changed(h), "values changed");
+    }
     void host_written() { fresh = Fresh::Host; }
 
     vbh_matrix_info info() {
@@ -899,6 +904,7 @@ void bind_vec(py::module& m, const std::string& name, bool is_vector) {
         .def_property_readonly("comm_calls", [](V&) { return 0L; })
         .def_property_readonly("device_ptr", [](V& v) { return reinterpret_cast<uintptr_t>(v.d()); })
         .def_property_readonly("ld", [](V& v) { return v.ld; })
+        .def_property_readonly("is_complex", [](V&) { return std::is_same_v<T, cplx>; })
         .def("mark_device_written", [](V& v) { v.buf.device_written(); })
         .def_buffer([](D& v) { return v.buffer(); });
     if (is_vector) {

@@ -683,8 +683,9 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
         hipStream_t s = (hipStream_t)stream;
         const int nbr = A->nbr;
         // ---- norms
-        double* an = (double*)dalloc(sizeof(double) * std::max<int64_t>(A->nnzb, 1));
-        double* bn = (double*)dalloc(sizeof(double) * std::max<int64_t>(B->nnzb, 1));
+        DevList tmp;  // every temporary of the call, freed on return or error
+        double* an = tmp.own((double*)dalloc(sizeof(double) * std::max<int64_t>(A->nnzb, 1)));
+        double* bn = tmp.own((double*)dalloc(sizeof(double) * std::max<int64_t>(B->nnzb, 1)));
         auto run_norms = [&](vbh_matrix* M, double* o) {
             if (M->nnzb == 0) return;
             if (M->dtype == VBH_F64)
@@ -714,11 +715,11 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
             VBH_REQUIRE(S <= (int64_t(1) << 30), "symbolic hash table too large");
             tsize[i] = int(S);
         }
-        int* d_tsize = upload(tsize, s);
-        int64_t* d_out_off = upload(out_off, s);
-        int* d_out_cols = (int*)dalloc(sizeof(int) * std::max<int64_t>(out_off[nbr], 1));
-        int* d_count = (int*)dalloc(sizeof(int) * std::max(nbr, 1));
-        int64_t* d_pairs = (int64_t*)dalloc(sizeof(int64_t) * std::max(nbr, 1));
+        int* d_tsize = tmp.own(upload(tsize, s));
+        int64_t* d_out_off = tmp.own(upload(out_off, s));
+        int* d_out_cols = tmp.own((int*)dalloc(sizeof(int) * std::max<int64_t>(out_off[nbr], 1)));
+        int* d_count = tmp.own((int*)dalloc(sizeof(int) * std::max(nbr, 1)));
+        int64_t* d_pairs = tmp.own((int64_t*)dalloc(sizeof(int64_t) * std::max(nbr, 1)));
         VBH_CHECK_HIP(hipMemsetAsync(d_pairs, 0, sizeof(int64_t) * std::max(nbr, 1), s));
         VBH_CHECK_HIP(hipMemsetAsync(d_count, 0, sizeof(int) * std::max(nbr, 1), s));
         SymArgs sa{};
@@ -745,11 +746,9 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
                 if (tsize[i] <= classes[q]) { c = q; break; }
             rows_cls[c].push_back(i);
         }
-        std::vector<void*> tmp_free;
         for (int q = 0; q < 3; ++q) {
             if (rows_cls[q].empty()) continue;
-            int* d_rows = upload(rows_cls[q], s);
-            tmp_free.push_back(d_rows);
+            int* d_rows = tmp.own(upload(rows_cls[q], s));
             sa.rows = d_rows;
             sa.nrows = int(rows_cls[q].size());
             const size_t shm = size_t(classes[q]) * (sizeof(double) + sizeof(int));
@@ -757,8 +756,7 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
             VBH_CHECK_HIP(hipGetLastError());
         }
         if (!rows_cls[3].empty() && B->nbc <= kNumMapCols && thr >= 0.0) {
-            int* d_rows = upload(rows_cls[3], s);
-            tmp_free.push_back(d_rows);
+            int* d_rows = tmp.own(upload(rows_cls[3], s));
             sa.rows = d_rows;
             sa.nrows = int(rows_cls[3].size());
             hipLaunchKernelGGL(k_symbolic_dense, dim3(sa.nrows), dim3(256), sizeof(double) * size_t(B->nbc), s, sa,
@@ -776,10 +774,11 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
                     goff.push_back(goff.back() + tsize[rows[pos]]);
                     ++pos;
                 }
-                int* d_rows = upload(batch, s);
-                int64_t* d_goff = upload(goff, s);
-                int* gk = (int*)dalloc(sizeof(int) * goff.back());
-                double* gv = (double*)dalloc(sizeof(double) * goff.back());
+                DevList bt;  // this batch's tables
+                int* d_rows = bt.own(upload(batch, s));
+                int64_t* d_goff = bt.own(upload(goff, s));
+                int* gk = bt.own((int*)dalloc(sizeof(int) * goff.back()));
+                double* gv = bt.own((double*)dalloc(sizeof(double) * goff.back()));
                 sa.rows = d_rows;
                 sa.nrows = int(batch.size());
                 sa.g_keys = gk;
@@ -788,7 +787,6 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
                 hipLaunchKernelGGL(k_symbolic<false>, dim3(sa.nrows), dim3(256), 0, s, sa);
                 VBH_CHECK_HIP(hipGetLastError());
                 VBH_CHECK_HIP(hipStreamSynchronize(s));
-                for (void* p : {(void*)d_rows, (void*)d_goff, (void*)gk, (void*)gv}) dfree(p);
             }
         }
         // ---- pattern of C: the per-row kept columns are compacted on the
@@ -807,27 +805,25 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
         const int64_t total = crp[nbr];
         std::vector<int> cci(total);
         if (total > 0) {
-            int* d_crp = upload(crp, s);
-            int* d_cci = (int*)dalloc(sizeof(int) * size_t(total));
+            DevList ct;
+            int* d_crp = ct.own(upload(crp, s));
+            int* d_cci = ct.own((int*)dalloc(sizeof(int) * size_t(total)));
             hipLaunchKernelGGL(k_compact_cols, dim3(nbr), dim3(256), 0, s, d_out_cols, d_out_off, d_crp, d_cci);
             VBH_CHECK_HIP(hipGetLastError());
             VBH_CHECK_HIP(hipMemcpyAsync(cci.data(), d_cci, sizeof(int) * size_t(total), hipMemcpyDeviceToHost, s));
             VBH_CHECK_HIP(hipStreamSynchronize(s));
-            dfree(d_crp);
-            dfree(d_cci);
         }
-        for (void* p : tmp_free) dfree(p);
-        for (void* p : {(void*)d_tsize, (void*)d_out_off, (void*)d_out_cols, (void*)d_count, (void*)d_pairs}) dfree(p);
         for (int i = 0; i < nbr; ++i)
             if (!std::is_sorted(cci.begin() + crp[i], cci.begin() + crp[i + 1]))
                 std::sort(cci.begin() + crp[i], cci.begin() + crp[i + 1]);
-        vbh_matrix* C = build_matrix(nbr, B->nbc, A->h_rbs.data(), B->h_cbs.data(), crp.data(), cci.data(), A->dtype, s);
-        unsigned long long* d_np = (unsigned long long*)dalloc(sizeof(unsigned long long));
+        MatrixGuard C(build_matrix(nbr, B->nbc, A->h_rbs.data(), B->h_cbs.data(), crp.data(), cci.data(), A->dtype, s));
+        unsigned long long* d_np = tmp.own((unsigned long long*)dalloc(sizeof(unsigned long long)));
+        vbh_matrix* Cm = C.m;
         VBH_CHECK_HIP(hipMemsetAsync(d_np, 0, sizeof(unsigned long long), s));
         // ---- numeric
-        if (C->nnzb > 0) {
+        if (Cm->nnzb > 0) {
             if (numeric == 2) {
-                std::vector<int> colstart(C->nnzb), width(nbr);
+                std::vector<int> colstart(Cm->nnzb), width(nbr);
                 for (int i = 0; i < nbr; ++i) {
                     int w = 0;
                     for (int q = crp[i]; q < crp[i + 1]; ++q) {
@@ -836,27 +832,25 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
                     }
                     width[i] = w;
                 }
-                int* d_cs = upload(colstart, s);
-                int* d_w = upload(width, s);
+                int* d_cs = tmp.own(upload(colstart, s));
+                int* d_w = tmp.own(upload(width, s));
                 MfmaArgs ma{A->row_ptr, A->col_ind, A->voff, (const double*)A->vals, an, A->rbs, A->cbs,
-                            B->row_ptr, B->col_ind, B->voff, (const double*)B->vals, bn, C->row_ptr, C->col_ind,
-                            C->voff, C->cbs, d_cs, d_w, (double*)C->vals, thr};
-                if (C->nbc <= kNumMapCols)
-                    hipLaunchKernelGGL(k_numeric_mfma_bf16_map, dim3(nbr), dim3(256), sizeof(int2) * size_t(C->nbc), s,
-                                       ma, C->nbc);
+                            B->row_ptr, B->col_ind, B->voff, (const double*)B->vals, bn, Cm->row_ptr, Cm->col_ind,
+                            Cm->voff, Cm->cbs, d_cs, d_w, (double*)Cm->vals, thr};
+                if (Cm->nbc <= kNumMapCols)
+                    hipLaunchKernelGGL(k_numeric_mfma_bf16_map, dim3(nbr), dim3(256), sizeof(int2) * size_t(Cm->nbc), s,
+                                       ma, Cm->nbc);
                 else
                     hipLaunchKernelGGL(k_numeric_mfma_bf16, dim3(nbr), dim3(256), 0, s, ma);
                 VBH_CHECK_HIP(hipGetLastError());
                 VBH_CHECK_HIP(hipStreamSynchronize(s));
-                dfree(d_cs);
-                dfree(d_w);
             } else {
                 auto launch = [&](auto* tag, auto* ctag) {
                     using T = std::remove_pointer_t<decltype(tag)>;
                     using CT = std::remove_pointer_t<decltype(ctag)>;
                     NumArgs<T, CT> na{A->row_ptr, A->col_ind, A->voff, (const T*)A->vals, an, A->rbs, A->cbs,
-                                      B->row_ptr, B->col_ind, B->voff, (const T*)B->vals, bn, C->row_ptr,
-                                      C->col_ind, C->voff, C->cbs, (T*)C->vals, d_np, thr,
+                                      B->row_ptr, B->col_ind, B->voff, (const T*)B->vals, bn, Cm->row_ptr,
+                                      Cm->col_ind, Cm->voff, Cm->cbs, (T*)Cm->vals, d_np, thr,
                                       nullptr, nullptr, nullptr};
                     if (A->max_block <= 8 && B->max_block <= 8) {
                         constexpr bool narrow_products = !std::is_same_v<T, CT>;
@@ -867,22 +861,21 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
                             ensure_vals32(B, s);
                             na.a_c = (const CT*)A->vals32;
                             na.b_c = (const CT*)B->vals32;
-                            na.c_c = (CT*)dalloc(sizeof(CT) * size_t(std::max<int64_t>(C->nnz, 1)));
-                            VBH_CHECK_HIP(hipMemsetAsync(na.c_c, 0, sizeof(CT) * size_t(C->nnz), s));
+                            na.c_c = tmp.own((CT*)dalloc(sizeof(CT) * size_t(std::max<int64_t>(Cm->nnz, 1))));
+                            VBH_CHECK_HIP(hipMemsetAsync(na.c_c, 0, sizeof(CT) * size_t(Cm->nnz), s));
                         }
-                        if (C->nbc <= kNumMapCols)
+                        if (Cm->nbc <= kNumMapCols)
                             hipLaunchKernelGGL((k_numeric_owned<T, CT, true>), dim3(nbr), dim3(64 * kNumWaves),
-                                               sizeof(int) * size_t(C->nbc), s, na, C->nbc);
+                                               sizeof(int) * size_t(Cm->nbc), s, na, Cm->nbc);
                         else
                             hipLaunchKernelGGL((k_numeric_owned<T, CT, false>), dim3(nbr), dim3(64 * kNumWaves), 0, s, na,
-                                               C->nbc);
+                                               Cm->nbc);
                         if constexpr (narrow_products) {
                             VBH_CHECK_HIP(hipGetLastError());
-                            hipLaunchKernelGGL((k_widen_values<T, CT>), dim3(unsigned((C->nnz + 255) / 256)), dim3(256),
-                                               0, s, na.c_c, (T*)C->vals, C->nnz);
+                            hipLaunchKernelGGL((k_widen_values<T, CT>), dim3(unsigned((Cm->nnz + 255) / 256)), dim3(256),
+                                               0, s, na.c_c, (T*)Cm->vals, Cm->nnz);
                             VBH_CHECK_HIP(hipGetLastError());
                             VBH_CHECK_HIP(hipStreamSynchronize(s));
-                            dfree(na.c_c);
                         }
                     } else {
                         hipLaunchKernelGGL((k_numeric<T, CT>), dim3(nbr), dim3(256), 0, s, na);
@@ -901,32 +894,23 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
         unsigned long long np = 0;
         VBH_CHECK_HIP(hipMemcpyAsync(&np, d_np, sizeof(np), hipMemcpyDeviceToHost, s));
         VBH_CHECK_HIP(hipStreamSynchronize(s));
-        dfree(d_np);
-        dfree(an);
-        dfree(bn);
-        const int64_t before = C->nnzb;
+        const int64_t before = C.m->nnzb;
         // ---- filter
         vbh_matrix* F = nullptr;
-        if (thr > 0.0 && C->nnzb > 0) {
-            const int rc = vbh_matrix_filter(C, thr, &F, stream);
-            if (rc != VBH_OK) {
-                destroy_matrix(C);
-                throw Error(rc, vbh_last_error());
-            }
+        if (thr > 0.0 && C.m->nnzb > 0) {
+            const int rc = vbh_matrix_filter(C.m, thr, &F, stream);
+            if (rc != VBH_OK) throw Error(rc, vbh_last_error());
         }
-        if (F) {
-            destroy_matrix(C);
-            C = F;
-        }
+        if (F) C.reset(F);
         if (stats) {
             int64_t sp = 0;
             for (int64_t p : pairs) sp += p;
             stats[0] = sp;
             stats[1] = int64_t(np);
             stats[2] = before;
-            stats[3] = C->nnzb;
+            stats[3] = C.m->nnzb;
         }
-        *out = C;
+        *out = C.release();
     }
 }
 
@@ -1003,9 +987,10 @@ static void complete_hermitian_impl(vbh_matrix_t U, vbh_matrix_t* out, hipStream
             std::vector<int> crow(C->nnzb);
             for (int i = 0; i < n; ++i)
                 for (int k = crp[i]; k < crp[i + 1]; ++k) crow[k] = i;
-            int* d_src = upload(src, s);
-            int* d_mode = upload(mode, s);
-            int* d_crow = upload(crow, s);
+            DevList tmp;
+            int* d_src = tmp.own(upload(src, s));
+            int* d_mode = tmp.own(upload(mode, s));
+            int* d_crow = tmp.own(upload(crow, s));
             const unsigned nb = unsigned((C->nnz + 255) / 256);
             if (U->dtype == VBH_F64)
                 hipLaunchKernelGGL(k_complete_hermitian<double>, dim3(nb), dim3(256), 0, s, C->nnz, C->voff,
@@ -1017,7 +1002,6 @@ static void complete_hermitian_impl(vbh_matrix_t U, vbh_matrix_t* out, hipStream
                                    (const c128*)U->vals, (c128*)C->vals);
             VBH_CHECK_HIP(hipGetLastError());
             VBH_CHECK_HIP(hipStreamSynchronize(s));
-            for (void* p : {(void*)d_src, (void*)d_mode, (void*)d_crow}) dfree(p);
         }
     } catch (...) {
         destroy_matrix(C);
@@ -1028,21 +1012,21 @@ static void complete_hermitian_impl(vbh_matrix_t U, vbh_matrix_t* out, hipStream
 
 extern "C" VBH_API int vbh_spgemm(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric, vbh_matrix_t* out,
                                   int64_t* stats, void* stream) {
-    return guard([&] { spgemm_impl(A, B, thr, numeric, 0, out, stats, stream); });
+    return guard_on(stream, [&] { spgemm_impl(A, B, thr, numeric, 0, out, stats, stream); });
 }
 
 extern "C" VBH_API int vbh_spgemm_upper(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric, vbh_matrix_t* out,
                                         int64_t* stats, void* stream) {
-    return guard([&] { spgemm_impl(A, B, thr, numeric, 1, out, stats, stream); });
+    return guard_on(stream, [&] { spgemm_impl(A, B, thr, numeric, 1, out, stats, stream); });
 }
 
 extern "C" VBH_API int vbh_matrix_complete_hermitian(vbh_matrix_t U, vbh_matrix_t* out, void* stream) {
-    return guard([&] { complete_hermitian_impl(U, out, (hipStream_t)stream); });
+    return guard_on(stream, [&] { complete_hermitian_impl(U, out, (hipStream_t)stream); });
 }
 
 extern "C" VBH_API int vbh_spgemm_hermitian(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
                                             vbh_matrix_t* out, int64_t* stats, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         vbh_matrix_t U = nullptr;
         spgemm_impl(A, B, thr, numeric, 1, &U, stats, stream);
         try {

@@ -177,7 +177,7 @@ extern "C" {
 
 VBH_API int vbh_vec_axpby(int dtype, int64_t rows, int K, const double* alpha, const void* x, int ldx,
                           const double* beta, void* y, int ldy, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(ldx >= K && ldy >= K && alpha && beta, "bad axpby arguments");
         if (rows * K == 0) return;
         dispatch_any(dtype, [&](auto* tag) {
@@ -191,7 +191,7 @@ VBH_API int vbh_vec_axpby(int dtype, int64_t rows, int K, const double* alpha, c
 }
 
 VBH_API int vbh_vec_scale(int dtype, int64_t rows, int K, const double* alpha, void* y, int ldy, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(ldy >= K && alpha, "bad scale arguments");
         if (rows * K == 0) return;
         dispatch_any(dtype, [&](auto* tag) {
@@ -204,7 +204,7 @@ VBH_API int vbh_vec_scale(int dtype, int64_t rows, int K, const double* alpha, v
 }
 
 VBH_API int vbh_vec_fill(int dtype, int64_t rows, int K, const double* value, void* y, int ldy, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(ldy >= K && value, "bad fill arguments");
         if (rows * K == 0) return;
         dispatch_any(dtype, [&](auto* tag) {
@@ -218,7 +218,7 @@ VBH_API int vbh_vec_fill(int dtype, int64_t rows, int K, const double* value, vo
 
 VBH_API int vbh_vec_pointwise(int dtype, int64_t rows, int K, const void* x, int ldx, int x_is_vector, void* y,
                               int ldy, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(ldy >= K && (x_is_vector || ldx >= K), "bad pointwise arguments");
         if (rows * K == 0) return;
         dispatch_any(dtype, [&](auto* tag) {
@@ -232,7 +232,7 @@ VBH_API int vbh_vec_pointwise(int dtype, int64_t rows, int K, const void* x, int
 
 VBH_API int vbh_vec_bdot(int dtype, int64_t rows, int K, const void* x, int ldx, const void* y, int ldy, double* out,
                          void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(ldx >= K && ldy >= K && out, "bad bdot arguments");
         hipStream_t s = (hipStream_t)stream;
         for (int e = 0; e < 2 * K; ++e) out[e] = 0.0;
@@ -259,7 +259,7 @@ VBH_API int vbh_vec_bdot(int dtype, int64_t rows, int K, const void* x, int ldx,
 
 VBH_API int vbh_vec_set_random(int dtype, int nblocks, const int* bs_dev, const int* gids_dev, const int64_t* offs_dev,
                                int K, uint64_t seed, int kind, double scale, void* y, int ldy, void* stream) {
-    return guard([&] {
+    return guard_on(stream, [&] {
         VBH_REQUIRE(ldy >= K, "ldy must be >= K");
         VBH_REQUIRE(kind >= 0 && kind <= 2, "kind must be 0 (Rademacher), 1 (random phase) or 2 (Gaussian)");
         VBH_REQUIRE(!(kind == 1 && (dtype == VBH_F64 || dtype == VBH_F32)),

@@ -69,6 +69,59 @@ def last_plan() -> Tuple[int, int, bool]:
     return tuple(_core.kpm_last_plan())
 
 
+def spectral_bounds(H: VBCSR, num_iter: int = 120, seed: int = 20250101) -> Tuple[float, float]:
+    """Extreme eigenvalues of a Hermitian H by Lanczos on the device.
+
+    Plain three-term Lanczos from a normalised Rademacher start vector (the
+    matrix-vector products and dots run through the library; only the
+    tridiagonal coefficients reach the host), then the extreme eigenvalues of
+    the tridiagonal matrix.  Ritz values lie inside the spectrum and converge
+    to its ends from within, so callers map them to a little less than +-1
+    (KPM scales to [-0.99, 0.99]); the reference examples estimate the bounds
+    by Gershgorin discs (examples/inp_nanocrystal_dos/run.py:156) or by hand
+    (examples/ex08_graphene_dos.py:72-76)."""
+    from scipy.linalg import eigh_tridiagonal
+
+    n = H.shape[0]
+    if n <= 1:
+        d = H.to_dense() if n else np.zeros((0, 0))
+        w = np.linalg.eigvalsh(d) if n else np.zeros(1)
+        return float(w[0]), float(w[-1])
+    v = H.create_vector()
+    v.set_random(seed, normalize=True)
+    v_prev = H.create_vector()
+    v_prev.set_constant(0.0)
+    w = H.create_vector()
+    alpha, beta = [], []
+    b = 0.0
+    for _ in range(min(num_iter, n)):
+        H.mult(v, w)
+        a = float(np.real(w.dot(v)))
+        w.axpby(-a, v, 1.0)  # w <- w - a v - b v_prev
+        if b != 0.0:
+            w.axpby(-b, v_prev, 1.0)
+        alpha.append(a)
+        b = float(np.sqrt(max(np.real(w.dot(w)), 0.0)))
+        if b <= 1e-12 * max(1.0, abs(a)):
+            break
+        beta.append(b)
+        w.scale(1.0 / b)
+        v_prev, v, w = v, w, v_prev
+    k = len(alpha)
+    ev = eigh_tridiagonal(np.asarray(alpha), np.asarray(beta[: k - 1]), eigvals_only=True)
+    return float(ev[0]), float(ev[-1])
+
+
+def scale_to_unit(H: VBCSR, edge: float = 0.99, bounds=None, num_iter: int = 120) -> Tuple[float, float]:
+    """Map H's spectrum onto [-edge, edge] in place (H <- (H - c)/s * edge via
+    scale and shift on the device); returns the bounds used."""
+    lo, hi = bounds if bounds is not None else spectral_bounds(H, num_iter=num_iter)
+    center, half = 0.5 * (lo + hi), 0.5 * (hi - lo)
+    H.shift(-center)
+    H.scale(edge / half)
+    return lo, hi
+
+
 def chebyshev_nodes(num_points: int) -> np.ndarray:
     """Ascending Gauss-Chebyshev nodes x_k = -cos(pi (k + 1/2)/K); never +-1."""
     return -np.cos(np.pi * (np.arange(num_points) + 0.5) / num_points)

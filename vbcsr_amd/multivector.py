@@ -10,21 +10,18 @@ from typing import Any, Union
 
 import numpy as np
 
-from ._wrapper_utils import core_buffer, duplicate_wrapper, infer_dtype_from_core, reduced_extent
+from ._device_backed import DeviceBacked
 from .vector import DistVector
 
 Scalar = Union[float, complex, int]
 
 
-class DistMultiVector:
+class DistMultiVector(DeviceBacked):
     def __init__(self, core_obj: Any, comm: Any = None):
-        self._core = core_obj
-        self.dtype = infer_dtype_from_core(core_obj)
-        self.comm = comm
-        self._global_rows = reduced_extent(comm, self.local_rows)
+        self._attach(core_obj, comm)
 
     def _local_buffer(self) -> np.ndarray:
-        return core_buffer(self._core)[: self.local_rows, :]
+        return self._host_view(self._core)[: self.local_rows, :]
 
     @property
     def local_rows(self) -> int:
@@ -44,9 +41,7 @@ class DistMultiVector:
 
     @property
     def shape(self):
-        if self._global_rows is not None:
-            return (self._global_rows, self.num_vectors)
-        return (None, self.num_vectors)
+        return (self.local_rows, self.num_vectors)
 
     @property
     def size(self) -> int:
@@ -92,9 +87,6 @@ class DistMultiVector:
 
     def reduce_ghosts(self) -> None:
         self._core.reduce_ghosts()
-
-    def copy(self) -> "DistMultiVector":
-        return duplicate_wrapper(self)
 
     def duplicate(self) -> "DistMultiVector":
         return DistMultiVector(self._core.duplicate(), self.comm)
@@ -206,9 +198,9 @@ class DistMultiVector:
         if np.isscalar(other):
             self._core.scale(1.0 / other)
         elif isinstance(other, DistMultiVector):
-            self._local_buffer()[:] /= core_buffer(other._core)[: other.local_rows, :]
+            self._local_buffer()[:] /= self._host_view(other._core)[: other.local_rows, :]
         elif isinstance(other, DistVector):
-            self._local_buffer()[:] /= core_buffer(other._core)[: other.local_size, None]
+            self._local_buffer()[:] /= self._host_view(other._core)[: other.local_size, None]
         elif isinstance(other, np.ndarray):
             self._local_buffer()[:] /= other
         else:

@@ -11,20 +11,17 @@ from typing import Any, Union
 
 import numpy as np
 
-from ._wrapper_utils import core_buffer, duplicate_wrapper, infer_dtype_from_core, reduced_extent
+from ._device_backed import DeviceBacked
 
 Scalar = Union[float, complex, int]
 
 
-class DistVector:
+class DistVector(DeviceBacked):
     def __init__(self, core_obj: Any, comm: Any = None):
-        self._core = core_obj
-        self.dtype = infer_dtype_from_core(core_obj)
-        self.comm = comm
-        self._global_size = reduced_extent(comm, self.local_size)
+        self._attach(core_obj, comm)
 
     def _local_buffer(self) -> np.ndarray:
-        return core_buffer(self._core)[: self.local_size]
+        return self._host_view(self._core)[: self.local_size]
 
     # -- sizes
     @property
@@ -45,7 +42,7 @@ class DistVector:
 
     @property
     def shape(self):
-        return (self._global_size,) if self._global_size is not None else (None,)
+        return (self.local_size,)
 
     @property
     def size(self) -> int:
@@ -95,9 +92,6 @@ class DistVector:
 
     def reduce_ghosts(self) -> None:
         self._core.reduce_ghosts()
-
-    def copy(self) -> "DistVector":
-        return duplicate_wrapper(self)
 
     def duplicate(self) -> "DistVector":
         return DistVector(self._core.duplicate(), self.comm)
@@ -201,7 +195,7 @@ class DistVector:
         if np.isscalar(other):
             self._core.scale(1.0 / other)
         elif isinstance(other, DistVector):
-            self._local_buffer()[:] /= core_buffer(other._core)[: other.local_size]
+            self._local_buffer()[:] /= self._host_view(other._core)[: other.local_size]
         elif isinstance(other, np.ndarray):
             self._local_buffer()[:] /= other
         else:
