@@ -157,15 +157,28 @@ VBH_API int vbh_spmm_fused(vbh_matrix_t m, int K, const void* X, int ldx, void* 
 VBH_API int vbh_kpm_moments(vbh_matrix_t m, int K, const void* V0, int ldv, int M, int mode,
                             double center, double half_width, int precision, double* moments,
                             void* stream);
+/* vbh_kpm_moments without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, results and errors as vbh_kpm_moments (the
+ * same kernels run); the work buffers (recursion vectors, partial dots,
+ * moments) live in a static device array of 128 MiB per GPU instead of
+ * library-allocated scratch.  A call needing more returns VBH_ERR_ARG with
+ * the size (about (3 or 4) * n_rows * K * sizeof(scalar) + 8 * steps *
+ * work-groups * 2 bytes).  The matrix's lazily built structures must exist:
+ * run one vbh_kpm_moments (or apply) at the same precision on it first, else
+ * VBH_ERR_STATE.  Calls in one process are serialised by a host lock (each
+ * synchronises its stream before returning), so calls never overlap on the
+ * array; the array keeps the last call's contents. */
+VBH_API int vbh_kpm_moments_noalloc(vbh_matrix_t m, int K, const void* V0, int ldv, int M, int mode,
+                                    double center, double half_width, int precision, double* moments,
+                                    void* stream);
 /* Batched KPM over many independent matrices in one launch per recursion
  * step, the steps captured in one HIP graph (graph != 0).  V0s[h] device
  * multivectors (n_rows_h x K, ld ldv[h]).  moments: host, count*M doubles. */
 VBH_API int vbh_kpm_moments_batched(int count, const vbh_matrix_t* ms, int K, const void* const* V0s,
                                     const int* ldv, int M, int mode, double center, double half_width,
                                     int precision, int graph, double* moments, void* stream);
-/* Chebyshev series Y = sum_n coef_n T_n(H~) X (complex coefficients as
- * interleaved doubles, nterms of them; real matrices use the real parts).
- * examples/ex09_graphene_dynamics.py:102-133, vbcsr/utils.py:71-126. */
 /* HIP-event duration (ms) of the step kernels of the last vbh_kpm_moments or
  * vbh_kpm_moments_batched call on this process, and the number of kernel
  * launches (graph nodes) it covers.  Profiling aid; no reference counterpart. */
@@ -186,6 +199,9 @@ VBH_API int vbh_set_batch_plan(int threshold, int parts);
 /* Plan of the last vbh_kpm_moments_batched call: matrix-core threshold, slot
  * partitions, and whether it came from timing (1) or the default rule (0). */
 VBH_API int vbh_kpm_last_plan(int* threshold, int* parts, int* tuned);
+/* Chebyshev series Y = sum_n coef_n T_n(H~) X (complex coefficients as
+ * interleaved doubles, nterms of them; real matrices use the real parts).
+ * examples/ex09_graphene_dynamics.py:102-133, vbcsr/utils.py:71-126. */
 VBH_API int vbh_cheb_apply(vbh_matrix_t m, int K, const void* X, int ldx, void* Y, int ldy,
                            int nterms, const double* coef, double center, double half_width,
                            int precision, void* stream);

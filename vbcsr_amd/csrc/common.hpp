@@ -37,6 +37,11 @@ void set_error(const std::string& msg);
         if (!(cond)) throw ::vbh::Error(VBH_ERR_ARG, (msg));      \
     } while (0)
 
+#define VBH_REQUIRE_STATE(cond, msg)                              \
+    do {                                                          \
+        if (!(cond)) throw ::vbh::Error(VBH_ERR_STATE, (msg));    \
+    } while (0)
+
 template <typename F>
 int guard(F&& f) {
     try {
@@ -281,6 +286,10 @@ T* upload(const std::vector<T>& h, hipStream_t s = nullptr) {
 void ensure_vals32(vbh_matrix* m, hipStream_t s);
 // Mark the device values as changed: the FP32 copy is rebuilt on next use.
 void values_changed(vbh_matrix* m);
+// Whether the FP32 copy is current / the row-class lists of threshold t exist
+// (so using them allocates nothing).
+bool has_vals32(vbh_matrix* m);
+bool has_row_classes(vbh_matrix* m, int t);
 // Ensure the adjoint (column-major incoming) plan exists.
 void ensure_adjoint_plan(vbh_matrix* m, hipStream_t s);
 // Row-class lists of matrix-core threshold t (block rows of t..16 rows on the

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <memory>
 
 #include "apply.hpp"
 
@@ -176,8 +177,9 @@ static size_t vec_bytes_of(int64_t N, int K) {
     return (sizeof(T) * size_t(std::max<int64_t>(N * K, 1)) + 255) & ~size_t(255);
 }
 
-// Assemble moments from the per-step dots d[s][0..K) (d1) and [K..2K) (d2),
-// K partial columns (1 when the step kernel already summed the columns).
+// Moments from the reduced per-step dots of one matrix, on the host (the
+// batched driver): the rule of k_assemble_moments below, d[s][0..2K) per step,
+// K partial columns.
 static void assemble_moments(int M, int mode, int K, double mu0, const double* d, double* mu) {
     for (int n = 0; n < M; ++n) mu[n] = 0.0;
     mu[0] = mu0;
@@ -201,100 +203,187 @@ static void assemble_moments(int M, int mode, int K, double mu0, const double* d
     }
 }
 
+__global__ void k_zero_f64(double* __restrict__ p, int64_t n) {
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
+}
+
+// Sum the per-work-group partials of every step: out[s*E + e] =
+// sum_{g < count} partials[(s*stride + g)*E + e]; grid (E, nsteps).
+__global__ void k_reduce_steps(const double* __restrict__ partials, int64_t stride, int count, int E,
+                               double* __restrict__ out) {
+    const int e = blockIdx.x;
+    const int64_t s = blockIdx.y;
+    const double* base = partials + s * stride * E;
+    double acc = 0.0;
+    for (int g = threadIdx.x; g < count; g += blockDim.x) acc += base[int64_t(g) * E + e];
+    __shared__ double red[256];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int m = 128; m >= 1; m >>= 1) {
+        if (threadIdx.x < m) red[threadIdx.x] += red[threadIdx.x + m];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[s * E + e] = red[0];
+}
+
+// Moments from the reduced dots (one thread): d[s][0..Kd) = d1 and
+// [Kd..2Kd) = d2 of step s (Kd = 1 when the step kernel summed the columns);
+// d0[0..K) = <t0, t0> per column.  Mode 0/1: mu_n = <v0, t_n> (step n-1
+// produced t_n).  Mode 2 (two moments per matvec, kpm.py:63-66): step 0 gives
+// mu_1 = <t0, t1>; step s >= 0 gives t_n (n = s + 1) and mu_2n = 2<t_n, t_n> -
+// mu_0, mu_2n-1 = 2<t_n, t_n-1> - mu_1.
+__global__ void k_assemble_moments(int M, int mode, int Kd, int K, int64_t nsteps, const double* __restrict__ d,
+                                   const double* __restrict__ d0, double* __restrict__ mu) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double mu0 = 0.0;
+    for (int v = 0; v < K; ++v) mu0 += d0[v];
+    for (int n = 0; n < M; ++n) mu[n] = 0.0;
+    mu[0] = mu0;
+    if (M == 1 || nsteps == 0) return;
+    auto sum = [&](int64_t st, int which) {
+        double t = 0.0;
+        for (int v = 0; v < Kd; ++v) t += d[st * 2 * Kd + which * Kd + v];
+        return t;
+    };
+    if (mode != 2) {
+        for (int n = 1; n < M; ++n) mu[n] = sum(n - 1, 0);
+        return;
+    }
+    mu[1] = sum(0, 1);
+    for (int64_t st = 0; st < nsteps; ++st) {
+        const int64_t n = st + 1;
+        const int64_t even = 2 * n, odd = 2 * n - 1;
+        if (even < M) mu[even] = 2.0 * sum(st, 0) - mu0;
+        if (odd > 1 && odd < M) mu[odd] = 2.0 * sum(st, 1) - mu[1];
+    }
+}
+
+// Device buffers of one single-matrix KPM call, laid out from one base
+// address (every region 256-byte aligned): the recursion vectors t_{n-1},
+// t_n, t_{n+1} (and v0 in modes 0/1), the per-step per-work-group partial dots,
+// the partials of mu_0 = <t0, t0>, the reduced dots and the moments.
+struct KpmLayout {
+    int64_t N = 0, nsteps = 0, rp = 0, rp0 = 0;
+    size_t vec_bytes = 0;
+    bool keep_v0 = false;
+    int Kd = 1, E = 2, G = 0, ndot = 0, ndot0 = 0;
+    size_t off_part = 0, off_part0 = 0, off_red = 0, off_red0 = 0, off_mu = 0, total = 0;
+};
+
+static size_t round256(size_t b) { return (b + 255) & ~size_t(255); }
+
+// Sizes and work-group counts of a call.  Reads the matrix's row-class lists
+// (built on first use).
+template <typename T, typename V>
+static KpmLayout kpm_layout(vbh_matrix* m, const V* vals, int K, int M, int mode) {
+    KpmLayout L;
+    L.N = m->n_rows;
+    L.nsteps = L.N > 0 ? step_count<T>(M, mode) : 0;
+    L.vec_bytes = vec_bytes_of<T>(L.N, K);
+    L.keep_v0 = mode != 2;  // modes 0/1 dot every t_n against v0
+    // fused steps sum their dots over the columns (E = 2); the plain path's
+    // dot kernel keeps one partial per column (E = 2K)
+    L.Kd = mode == 0 ? K : 1;
+    L.E = 2 * L.Kd;
+    L.G = m->G;
+    if (L.nsteps > 0 && mode != 0) {
+        auto probe = make_args<T, V>(m, vals, K, (const T*)nullptr, K, (T*)nullptr, K);
+        probe.dot_cols = 1;
+        L.G = apply_groups(m, probe);
+    }
+    L.rp = std::max<int64_t>(256, (L.N + L.G - 1) / std::max(L.G, 1));
+    L.ndot = int((L.N + L.rp - 1) / L.rp);
+    L.rp0 = std::max<int64_t>(256, (L.N + 1023) / 1024);
+    L.ndot0 = int((L.N + L.rp0 - 1) / L.rp0);
+    L.off_part = (L.keep_v0 ? 4 : 3) * L.vec_bytes;
+    L.off_part0 = L.off_part + round256(sizeof(double) * size_t(L.nsteps) * L.G * L.E);
+    L.off_red = L.off_part0 + round256(sizeof(double) * size_t(std::max(L.ndot0, 1)) * K);
+    L.off_red0 = L.off_red + round256(sizeof(double) * size_t(L.nsteps) * L.E);
+    L.off_mu = L.off_red0 + round256(sizeof(double) * K);
+    L.total = L.off_mu + round256(sizeof(double) * size_t(M));
+    return L;
+}
+
+// One single-matrix KPM call on the buffers at base (L.total bytes of device
+// memory): moments (host, M doubles) of H~ = (H - center)/half.
 template <typename T, typename S, typename V>
-static void kpm_single(vbh_matrix* m, int K, const S* V0, int ldv, int M, int mode, double center, double half,
-                       double* moments, hipStream_t s) {
-    const int64_t N = m->n_rows;
-    const int64_t nsteps = step_count<T>(M, mode);
-    const size_t vec_bytes = vec_bytes_of<T>(N, K);
-    const bool keep_v0 = mode != 2;  // modes 0/1 dot every t_n against v0
-    T* buf = (T*)scratch(3 * vec_bytes + (keep_v0 ? vec_bytes : 0), 4);
-    T* t[3] = {buf, (T*)((char*)buf + vec_bytes), (T*)((char*)buf + 2 * vec_bytes)};
-    T* v0 = keep_v0 ? (T*)((char*)buf + 3 * vec_bytes) : t[0];
+static void kpm_single(vbh_matrix* m, const V* vals, int K, const S* V0, int ldv, int M, int mode, double center,
+                       double half, double* moments, hipStream_t s, const KpmLayout& L, char* base, bool timed) {
+    const int64_t N = L.N, nsteps = L.nsteps;
+    const size_t vb = L.vec_bytes;
+    T* t[3] = {(T*)base, (T*)(base + vb), (T*)(base + 2 * vb)};
+    T* v0 = L.keep_v0 ? (T*)(base + 3 * vb) : t[0];
+    double* partials = (double*)(base + L.off_part);
+    double* part0 = (double*)(base + L.off_part0);
+    double* red = (double*)(base + L.off_red);
+    double* red0 = (double*)(base + L.off_red0);
+    double* mu = (double*)(base + L.off_mu);
+    const int G = L.G, E = L.E;
     const unsigned nb = unsigned((N * K + 255) / 256);
     if (N * K > 0) {
         hipLaunchKernelGGL((k_copy_convert<S, T>), dim3(nb), dim3(256), 0, s, N, K, V0, int64_t(ldv), t[0], int64_t(K));
         VBH_CHECK_HIP(hipGetLastError());
-        if (keep_v0) VBH_CHECK_HIP(hipMemcpyAsync(v0, t[0], sizeof(T) * N * K, hipMemcpyDeviceToDevice, s));
-    }
-    // mu_0 = sum_v <t0, t0>
-    std::vector<double> dd(2 * K);
-    {
-        const int dt = std::is_same_v<T, double> ? VBH_F64 : std::is_same_v<T, float> ? VBH_F32
-                       : std::is_same_v<T, c128> ? VBH_C128 : VBH_C64;
-        if (vbh_vec_bdot(dt, N, K, t[0], K, t[0], K, dd.data(), s) != VBH_OK) throw Error(VBH_ERR_HIP, vbh_last_error());
-    }
-    double mu0 = 0.0;
-    for (int v = 0; v < K; ++v) mu0 += dd[2 * v];
-    if (M <= 1 || N == 0) {
-        for (int n = 0; n < M; ++n) moments[n] = 0.0;
-        if (M >= 1) moments[0] = mu0;
-        return;
-    }
-    const V* vals = compute_values<V>(m, s);
-    // fused steps sum their dots over the columns (E = 2); the plain path's
-    // dot kernel keeps one partial per column (E = 2K)
-    const int Kd = mode == 0 ? K : 1;
-    const int E = 2 * Kd;
-    int G = m->G;
-    if (mode != 0) {
-        auto probe = make_args<T, V>(m, vals, K, t[0], K, t[1], K);
-        probe.dot_cols = 1;
-        G = apply_groups(m, probe);
-    }
-    double* partials = (double*)scratch(sizeof(double) * nsteps * G * E, 5);
-    const double a1 = 1.0 / half, b1 = -center / half;  // t1 = H~ t0
-    const double a2 = 2.0 / half, b2 = -2.0 * center / half;  // t_{n+1} = 2 H~ t_n - t_{n-1}
-    const int64_t rp = std::max<int64_t>(256, (N + G - 1) / G);
-    const int ndot = int((N + rp - 1) / rp);
-    if (mode == 0) {
-        VBH_REQUIRE(ndot <= G, "internal: dot partial blocks exceed work-groups");
-        VBH_CHECK_HIP(hipMemsetAsync(partials, 0, sizeof(double) * nsteps * G * E, s));
-    }
-    StepTimer timer(s);
-    timer.start();
-    if (mode == 0) {
-        // plain: SpMM kernel, then a vector pass, then a dot kernel per step
-        for (int64_t st = 0; st < nsteps; ++st) {
-            T* X = t[st % 3];
-            T* Y = t[(st + 1) % 3];
-            T* Z = t[(st + 2) % 3];
-            apply_plain<T, V>(m, vals, K, X, K, Y, K, s);
-            hipLaunchKernelGGL(k_axpbypcz<T>, dim3(nb), dim3(256), 0, s, N * K, st == 0 ? a1 : a2, Y,
-                               st == 0 ? b1 : b2, (const T*)X, st == 0 ? 0.0 : -1.0, (const T*)Z);
-            VBH_CHECK_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_redot_partials<T>, dim3(ndot), dim3(256), 0, s, N, K, (const T*)v0,
-                               (const T*)Y, rp, partials + st * G * E, E, 0);
-            VBH_CHECK_HIP(hipGetLastError());
-        }
+        if (L.keep_v0) VBH_CHECK_HIP(hipMemcpyAsync(v0, t[0], sizeof(T) * N * K, hipMemcpyDeviceToDevice, s));
+        // mu_0 = sum_v <t0, t0>
+        hipLaunchKernelGGL(k_redot_partials<T>, dim3(L.ndot0), dim3(256), 0, s, N, K, (const T*)t[0],
+                           (const T*)t[0], L.rp0, part0, K, 0);
+        VBH_CHECK_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_reduce_steps, dim3(K, 1), dim3(256), 0, s, part0, int64_t(0), L.ndot0, K, red0);
+        VBH_CHECK_HIP(hipGetLastError());
     } else {
-        for (int64_t st = 0; st < nsteps; ++st) {
-            auto a = make_args<T, V>(m, vals, K, t[st % 3], K, t[(st + 1) % 3], K);
-            a.alpha = st == 0 ? a1 : a2;
-            a.beta = st == 0 ? b1 : b2;
-            a.gamma = st == 0 ? 0.0 : -1.0;
-            a.Z = t[(st + 2) % 3];
-            a.ldz = K;
-            a.W = mode == 1 ? v0 : nullptr;
-            a.ldw = K;
-            a.dot_cols = 1;
-            a.partials = partials + st * G * E;
-            launch_rows<T, true, V>(m, a, s);
-        }
+        hipLaunchKernelGGL(k_zero_f64, dim3(1), dim3(64), 0, s, red0, int64_t(K));
+        VBH_CHECK_HIP(hipGetLastError());
     }
-    timer.stop(mode == 0 ? 3 * nsteps : nsteps);
-    // one reduction over all steps
-    double* red = (double*)scratch(sizeof(double) * nsteps * E, 6);
-    const int hseg[2] = {0, G};
-    int* dseg = (int*)scratch(sizeof(int) * 2, 7);
-    VBH_CHECK_HIP(hipMemcpyAsync(dseg, hseg, sizeof(int) * 2, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_reduce_segments, dim3(E, 1, unsigned(nsteps)), dim3(256), 0, s, partials, G, E, dseg, 1, red,
-                       (const int*)nullptr, 0);
+    if (nsteps > 0) {
+        const double a1 = 1.0 / half, b1 = -center / half;        // t1 = H~ t0
+        const double a2 = 2.0 / half, b2 = -2.0 * center / half;  // t_{n+1} = 2 H~ t_n - t_{n-1}
+        std::unique_ptr<StepTimer> timer;
+        if (timed) {
+            timer.reset(new StepTimer(s));
+            timer->start();
+        }
+        if (mode == 0) {
+            // plain: SpMM kernel, then a vector pass, then a dot kernel per step
+            VBH_REQUIRE(L.ndot <= G, "internal: dot partial blocks exceed work-groups");
+            for (int64_t st = 0; st < nsteps; ++st) {
+                T* X = t[st % 3];
+                T* Y = t[(st + 1) % 3];
+                T* Z = t[(st + 2) % 3];
+                apply_plain<T, V>(m, vals, K, X, K, Y, K, s);
+                hipLaunchKernelGGL(k_axpbypcz<T>, dim3(nb), dim3(256), 0, s, N * K, st == 0 ? a1 : a2, Y,
+                                   st == 0 ? b1 : b2, (const T*)X, st == 0 ? 0.0 : -1.0, (const T*)Z);
+                VBH_CHECK_HIP(hipGetLastError());
+                hipLaunchKernelGGL(k_redot_partials<T>, dim3(L.ndot), dim3(256), 0, s, N, K, (const T*)v0,
+                                   (const T*)Y, L.rp, partials + st * G * E, E, 0);
+                VBH_CHECK_HIP(hipGetLastError());
+            }
+        } else {
+            for (int64_t st = 0; st < nsteps; ++st) {
+                auto a = make_args<T, V>(m, vals, K, t[st % 3], K, t[(st + 1) % 3], K);
+                a.alpha = st == 0 ? a1 : a2;
+                a.beta = st == 0 ? b1 : b2;
+                a.gamma = st == 0 ? 0.0 : -1.0;
+                a.Z = t[(st + 2) % 3];
+                a.ldz = K;
+                a.W = mode == 1 ? v0 : nullptr;
+                a.ldw = K;
+                a.dot_cols = 1;
+                a.partials = partials + st * G * E;
+                launch_rows<T, true, V>(m, a, s);
+            }
+        }
+        if (timer) timer->stop(mode == 0 ? 3 * nsteps : nsteps);
+        // one reduction over all steps (the plain path's dot kernel fills
+        // the first ndot partial rows of a step, the fused kernels all G)
+        hipLaunchKernelGGL(k_reduce_steps, dim3(E, unsigned(nsteps)), dim3(256), 0, s, partials, int64_t(G),
+                           mode == 0 ? L.ndot : G, E, red);
+        VBH_CHECK_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_assemble_moments, dim3(1), dim3(64), 0, s, M, mode, L.Kd, K, nsteps, (const double*)red,
+                       (const double*)red0, mu);
     VBH_CHECK_HIP(hipGetLastError());
-    std::vector<double> d(nsteps * E);
-    VBH_CHECK_HIP(hipMemcpyAsync(d.data(), red, sizeof(double) * nsteps * E, hipMemcpyDeviceToHost, s));
+    VBH_CHECK_HIP(hipMemcpyAsync(moments, mu, sizeof(double) * M, hipMemcpyDeviceToHost, s));
     VBH_CHECK_HIP(hipStreamSynchronize(s));
-    assemble_moments(M, mode, Kd, mu0, d.data(), moments);
 }
 
 // f(T*, S*, V*): recursion vector type T, start-vector (input) type S, matrix
@@ -725,22 +814,97 @@ static void cheb_apply(vbh_matrix* m, int K, const T* X, int ldx, T* Y, int ldy,
 
 using namespace vbh;
 
+static void check_kpm_args(const vbh_matrix* m, int K, const void* V0, int ldv, int M, int mode, double half_width,
+                           const double* moments) {
+    VBH_REQUIRE(m && V0 && moments, "null argument");
+    VBH_REQUIRE(m->nbc == m->nbr, "KPM needs a square local pattern without ghosts");
+    VBH_REQUIRE(mode >= 0 && mode <= 2, "mode must be 0, 1 or 2");
+    VBH_REQUIRE(K >= 1 && K <= 64 && ldv >= K, "KPM supports 1 <= K <= 64 columns per call, ldv >= K");
+    VBH_REQUIRE(half_width > 0.0, "half_width must be positive");
+    (void)M;
+}
+
+// Scratch of vbh_kpm_moments_noalloc: a static device array, one copy per
+// GPU, present from the library's load until the process ends.  Host code
+// never passes the array's name to a kernel or a copy (on the host it is only
+// a stand-in symbol); it uses the device address that hipGetSymbolAddress
+// returns for the current device, looked up once per device.
+constexpr size_t kNoAllocScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+static std::mutex g_noalloc_mu;
+
+static void* noalloc_scratch_ptr() {
+    static std::mutex mu;
+    static std::map<int, void*> ptrs;
+    int dev = 0;
+    VBH_CHECK_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = ptrs.find(dev);
+    if (it != ptrs.end()) return it->second;
+    void* p = nullptr;
+    VBH_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch)));
+    VBH_REQUIRE(p != nullptr, "internal: no device address for the static scratch");
+    ptrs[dev] = p;
+    return p;
+}
+
 extern "C" {
 
 VBH_API int vbh_kpm_moments(vbh_matrix_t m, int K, const void* V0, int ldv, int M, int mode, double center,
                             double half_width, int precision, double* moments, void* stream) {
     return guard_on(stream, [&] {
-        VBH_REQUIRE(m && V0 && moments, "null argument");
-        VBH_REQUIRE(m->nbc == m->nbr, "KPM needs a square local pattern without ghosts");
-        VBH_REQUIRE(mode >= 0 && mode <= 2, "mode must be 0, 1 or 2");
-        VBH_REQUIRE(K >= 1 && K <= 64 && ldv >= K, "KPM supports 1 <= K <= 64 columns per call, ldv >= K");
-        VBH_REQUIRE(half_width > 0.0, "half_width must be positive");
+        check_kpm_args(m, K, V0, ldv, M, mode, half_width, moments);
         if (M <= 0) return;
         dispatch_kpm(m, precision, [&](auto* t, auto* sty, auto* vty) {
             using T = std::remove_pointer_t<decltype(t)>;
             using S = std::remove_pointer_t<decltype(sty)>;
             using V = std::remove_pointer_t<decltype(vty)>;
-            kpm_single<T, S, V>(m, K, (const S*)V0, ldv, M, mode, center, half_width, moments, (hipStream_t)stream);
+            hipStream_t s = (hipStream_t)stream;
+            const bool steps = m->n_rows > 0 && step_count<T>(M, mode) > 0;
+            const V* vals = steps ? compute_values<V>(m, s) : nullptr;
+            const KpmLayout L = kpm_layout<T, V>(m, vals, K, M, mode);
+            char* base = (char*)vbh::scratch(L.total, 4);
+            kpm_single<T, S, V>(m, vals, K, (const S*)V0, ldv, M, mode, center, half_width, moments, s, L, base, true);
+        });
+    });
+}
+
+VBH_API int vbh_kpm_moments_noalloc(vbh_matrix_t m, int K, const void* V0, int ldv, int M, int mode, double center,
+                                    double half_width, int precision, double* moments, void* stream) {
+    return guard([&] {
+        check_kpm_args(m, K, V0, ldv, M, mode, half_width, moments);
+        if (M <= 0) return;
+        char* base = (char*)noalloc_scratch_ptr();
+        std::lock_guard<std::mutex> lk(g_noalloc_mu);  // one call per process at a time owns the array
+        dispatch_kpm(m, precision, [&](auto* t, auto* sty, auto* vty) {
+            using T = std::remove_pointer_t<decltype(t)>;
+            using S = std::remove_pointer_t<decltype(sty)>;
+            using V = std::remove_pointer_t<decltype(vty)>;
+            hipStream_t s = (hipStream_t)stream;
+            const bool steps = m->n_rows > 0 && step_count<T>(M, mode) > 0;
+            const V* vals = nullptr;
+            if (steps) {
+                // the matrix's lazily built structures must exist already:
+                // building them allocates
+                VBH_REQUIRE_STATE(has_row_classes(m, kMfmaMinRows),
+                                  "matrix not prepared: its row-class lists are built by its first apply or KPM "
+                                  "call (which allocates); run one before vbh_kpm_moments_noalloc");
+                if constexpr (std::is_same_v<V, float> || std::is_same_v<V, c64>) {
+                    VBH_REQUIRE_STATE(has_vals32(m),
+                                      "matrix not prepared: the FP32 value copy is built by its first apply or KPM "
+                                      "call at precision 1 or 2 (which allocates); run one before "
+                                      "vbh_kpm_moments_noalloc");
+                }
+                vals = compute_values<V>(m, s);
+            }
+            const KpmLayout L = kpm_layout<T, V>(m, vals, K, M, mode);
+            if (L.total > kNoAllocScratchBytes)
+                throw Error(VBH_ERR_ARG, "vbh_kpm_moments_noalloc: this call needs " + std::to_string(L.total) +
+                                             " bytes of scratch; the static scratch holds " +
+                                             std::to_string(kNoAllocScratchBytes) +
+                                             " (use vbh_kpm_moments, or fewer rows, columns or moments)");
+            kpm_single<T, S, V>(m, vals, K, (const S*)V0, ldv, M, mode, center, half_width, moments, s, L, base,
+                                false);
         });
     });
 }

@@ -471,6 +471,16 @@ void values_changed(vbh_matrix* m) {
     m->vals32_valid = false;
 }
 
+bool has_vals32(vbh_matrix* m) {
+    std::lock_guard<std::mutex> lk(m->lazy_mu);
+    return m->vals32_valid;
+}
+
+bool has_row_classes(vbh_matrix* m, int t) {
+    std::lock_guard<std::mutex> lk(m->lazy_mu);
+    return m->row_lists.count(t) != 0;
+}
+
 // Column-major "incoming" plan for the adjoint: for every local column j the
 // (row, slot) pairs in ascending row order -- one writer per output block, as
 // in the reference's adjoint plan (vbcsr_backend.hpp:543-625).
