@@ -23,6 +23,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import subprocess
 import os
 import re
 import sys
@@ -48,6 +49,9 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--quick", action="store_true", help="smaller lattice (smoke of the bench itself)")
     p.add_argument("--only", default="", help="comma list: kpm,spmv,batched,prop,mcweeny,cpu (default all)")
+    p.add_argument("--mcweeny-iters", type=int, default=5,
+                   help="McWeeny iterations per precision (the configured 10 fill P to ~0.9 G values and take "
+                        "minutes per precision; 5 keep the default run short)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a 1 GiB device-to-device copy 5 times (scale for the HBM byte counters)")
     return p.parse_args()
@@ -100,6 +104,27 @@ def read_traffic(name):
         return None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", open(path).read())
     return float(m.group(1)) if m else None
+
+
+def host_info():
+    """CPU model and topology of the host the CPU baselines run on (lscpu)."""
+    info = {}
+    try:
+        txt = subprocess.run(["lscpu"], stdout=subprocess.PIPE, stderr=subprocess.DEVNULL, text=True,
+                             timeout=10).stdout
+        for line in txt.splitlines():
+            k, _, v = line.partition(":")
+            if k.strip() in ("Model name", "CPU(s)", "Socket(s)", "Core(s) per socket", "Thread(s) per core",
+                             "NUMA node(s)", "CPU max MHz", "L3"):
+                info[k.strip()] = v.strip()
+    except Exception:  # lscpu missing: fall back to /proc/cpuinfo
+        try:
+            models = [ln.split(":", 1)[1].strip() for ln in open("/proc/cpuinfo") if ln.startswith("model name")]
+            info = {"Model name": models[0] if models else "?", "CPU(s)": str(len(models))}
+        except OSError:
+            pass
+    info["threads_used"] = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return info
 
 
 def cpu_baseline(G, K, M, seed):
@@ -197,23 +222,87 @@ def bench_propagation(args, sync, log):
     return res
 
 
+def bench_kpm_scaled(args, sync, log, nx=800):
+    """The C2 KPM step on a lattice large enough that one step's bytes (about
+    1.5 GB at FP32) exceed the 256 MB Infinity Cache several times: graphene
+    nx x nx cells, spectrum scaled on the device, 200 moments x 10 vectors;
+    FP32 and FP64 fused steps, per-launch time from HIP events."""
+    import vbcsr_amd
+    from vbcsr_amd import VBCSR
+    from vbcsr_amd import kpm as KP
+    from vbcsr_amd import workloads as W
+
+    t0 = time.perf_counter()
+    G = W.graphene(nx, nx, 4, seed=0, scale=False)
+    H = VBCSR.from_block_csr(G.bs, G.row_ptr, G.col_ind, G.vals)
+    KP.scale_to_unit(H, 0.99)
+    V0 = H.create_multivector(10)
+    V0.set_random(42, normalize=False)
+    sync()
+    log(f"graphene {nx}x{nx} built and scaled in {time.perf_counter() - t0:.1f} s")
+    res = {"workload": f"graphene {nx}x{nx} cells, 4 orbitals (N={G.n}), 200 moments x 10 vectors, two moments "
+                       f"per matvec"}
+    for name, prec, p in (("fp32", "fp32", 4), ("fp64", "fp64", 8)):
+        KP.kpm_moments(H, V0, 200, mode="two", precision=prec)
+        sync()
+        KP.kpm_moments(H, V0, 200, mode="two", precision=prec)
+        kms, launches = vbcsr_amd._core.kpm_last_step_time()
+        per = kms / max(launches, 1)
+        alg = spmm_bytes(G, 10, p, p, with_z=True)
+        gbs = alg / (per * 1e-3) / 1e9
+        res[name] = {"avg_launch_us": per * 1e3, "alg_bytes_per_launch": alg, "achieved_gbs": gbs,
+                     "frac_hbm": gbs / HBM_PEAK_GBS}
+        log(f"kpm {nx}x{nx} {name}: {per * 1e3:.1f} us per step, {gbs:.0f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}%)")
+    res["fp32_over_fp64_step_speed"] = res["fp64"]["avg_launch_us"] / res["fp32"]["avg_launch_us"]
+    return res
+
+
+def mcweeny_reference_cpu(P0, thr, iters, budget, log):
+    """The reference C++ core (oracle/_ref, OpenMP) on the same P0: the
+    README/ex07 iteration P2 = P.spmm(P), P3 = P2.spmm(P), P = 3 P2 - 2 P3
+    (BlockSpMat::spmm and axpby), iterations run until the time budget is
+    spent; per-iteration seconds."""
+    import oracle
+
+    if not oracle.ref_available():
+        return None
+    P = oracle.RefMat(oracle.Mat.of(P0))
+    per = []
+    for it in range(iters):
+        t0 = time.perf_counter()
+        P2 = P.spmm(P, thr)
+        P3 = P2.spmm(P, thr)
+        Pn = P2.duplicate()
+        Pn.axpby(-2.0, P3, 3.0)  # self = -2 P3 + 3 P2
+        P = Pn
+        per.append(time.perf_counter() - t0)
+        log(f"mcweeny reference CPU iteration {it}: {per[-1]:.2f} s, {P.nnzb} blocks")
+        if sum(per) > budget:
+            break
+    return per
+
+
 def bench_mcweeny(args, sync, log):
     """C3: McWeeny purification P <- 3P^2 - 2P^3 with thresholded SpGEMM
     (threshold 1e-5), 5000 block rows of sizes 4..8, ~30 neighbours per row,
-    P0 = 0.1 H + 0.5 I with H normalised by its Gershgorin bound (as written,
-    without it, P0's spectrum leaves [-0.5, 1.5] and the iteration diverges);
-    numeric phase in FP64 (full and Hermitian products), FP32 and BF16
-    (matrix cores); idempotency
-    ||P^2 - P||_F and trace in FP64.  An iteration that passes the time budget
-    ends the run for that precision (reported)."""
+    P0 = 0.1 H + 0.5 I with H divided by its largest eigenvalue magnitude
+    (Lanczos; as written, without it, P0's spectrum leaves [-0.5, 1.5] and the
+    iteration diverges), so P0's spectrum spans [0.4, 0.6].  Numeric phase in
+    FP64 (full and Hermitian products), FP32 and BF16 (matrix cores);
+    idempotency ||P^2 - P||_F and trace in FP64, and each leg's P against the
+    FP64 leg's.  The reference C++ core runs the first iterations on the host
+    CPU for comparison.  An iteration that passes the time budget ends the run
+    for that precision (reported)."""
     from vbcsr_amd import VBCSR
     from vbcsr_amd import workloads as W
 
     nb = 1000 if args.quick else 5000
-    iters, budget = 10, 60.0
-    P0 = W.mcweeny_start(nb=nb, seed=3, normalize=True)
-    res = {"workload": f"{nb} block rows, blocks 4-8, ~30 neighbours, Gershgorin-normalised H, threshold 1e-5, "
-                       f"up to {iters} iterations"}
+    iters, budget, thr = args.mcweeny_iters, 300.0, 1e-5
+    P0 = W.mcweeny_start(nb=nb, seed=3, normalize="spectral")
+    lo, hi = P0.meta["spectrum"]
+    res = {"workload": f"{nb} block rows, blocks 4-8, ~30 neighbours, H / max|eig H| (Lanczos: [{lo:.3f}, {hi:.3f}]),"
+                       f" P0 = 0.1 H + 0.5 I, threshold {thr:g}, up to {iters} iterations"}
+    P_ref = None
     # "fp64_hermitian": both products by spmm_hermitian (P, P^2 and P^3 are
     # symmetric: only the upper block triangle is multiplied, then mirrored)
     for leg in ("fp64", "fp64_hermitian", "fp32", "bf16"):
@@ -223,30 +312,55 @@ def bench_mcweeny(args, sync, log):
         sync()
         t0 = time.perf_counter()
         done = 0
-        per_iter = []
+        per_iter, stats = [], []
         for it in range(iters):
             ti = time.perf_counter()
             if herm:
-                P2 = P.spmm_hermitian(P, 1e-5, numeric=numeric)
-                P3 = P2.spmm_hermitian(P, 1e-5, numeric=numeric)
+                P2 = P.spmm_hermitian(P, thr, numeric=numeric)
+                s1 = P.spmm_stats
+                P3 = P2.spmm_hermitian(P, thr, numeric=numeric)
             else:
-                P2 = P.spmm(P, 1e-5, numeric=numeric)
-                P3 = P2.spmm(P, 1e-5, numeric=numeric)
+                P2 = P.spmm(P, thr, numeric=numeric)
+                s1 = P.spmm_stats
+                P3 = P2.spmm(P, thr, numeric=numeric)
+            s2 = P2.spmm_stats
             P = P2 * 3.0 - P3 * 2.0
             sync()
             per_iter.append(time.perf_counter() - ti)
+            stats.append({"symbolic_pairs": int(s1[0] + s2[0]), "numeric_pairs": int(s1[1] + s2[1]),
+                          "blocks_P2": int(s1[3]), "blocks_P3": int(s2[3])})
             done += 1
-            log(f"mcweeny {leg} iteration {it}: {per_iter[-1] * 1e3:.1f} ms, {P.nnz} values")
-            if time.perf_counter() - t0 > budget:
+            log(f"mcweeny {leg} iteration {it}: {per_iter[-1] * 1e3:.1f} ms, {P.nnz} values, "
+                f"numeric pairs {stats[-1]['numeric_pairs']}")
+            if time.perf_counter() - t0 > budget:  # safety stop only; reported as done < iters
                 break
         secs = sum(per_iter)  # the iterations only (not the logging between them)
         P2 = P.spmm(P, 0.0)  # FP64 check: ||P^2 - P||_F over the union pattern, and tr P
         idem = float(np.linalg.norm((P2 - P).get_values()))
+        del P2
         res[leg] = {"iterations": done, "seconds": secs, "ms_per_iteration": [t * 1e3 for t in per_iter],
-                    "trace": float(P.trace()), "values_final": int(len(P.get_values())),
+                    "spgemm_stats_per_iteration": stats,
+                    "trace": float(P.trace()), "values_final": int(P.nnz),
                     "idempotency_frobenius": idem}
+        if P_ref is None:
+            P_ref = P
+        else:
+            D = (P - P_ref).get_values()
+            res[leg]["vs_fp64"] = {"max_abs": float(np.abs(D).max()) if D.size else 0.0,
+                                   "frobenius": float(np.linalg.norm(D)),
+                                   "trace_diff": res[leg]["trace"] - res["fp64"]["trace"]}
         log(f"mcweeny {leg}: {done} iterations in {secs:.2f} s, trace {res[leg]['trace']:.6f}, "
-            f"||P^2-P||_F {idem:.3e}")
+            f"||P^2-P||_F {idem:.3e}" + (f", vs fp64 max {res[leg]['vs_fp64']['max_abs']:.2e}"
+                                          if "vs_fp64" in res[leg] else ""))
+    if not args.no_cpu and not args.quick:
+        cpu = mcweeny_reference_cpu(P0, thr, iters, 12.0, log)
+        if cpu:
+            n = len(cpu)
+            gpu_same = sum(res["fp64"]["ms_per_iteration"][:n]) / 1e3
+            res["cpu_reference"] = {"iterations": n, "seconds": sum(cpu), "s_per_iteration": cpu,
+                                    "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                                    "gpu_fp64_seconds_same_iterations": gpu_same,
+                                    "gpu_speedup_same_iterations": sum(cpu) / gpu_same if gpu_same > 0 else None}
     return res
 
 
@@ -324,6 +438,8 @@ def main():
         pl64 = p64["kernel_ms"] / max(p64["launches"], 1)
         a64 = spmm_bytes(G, K, 8, 8, with_z=True)
         out["fp64_fused_step_gbs"] = a64 / (pl64 * 1e-3) / 1e9
+        if not args.quick:
+            out["kpm_hbm_resident"] = bench_kpm_scaled(args, sync, log)
 
     if want("spmv"):
         # block SpMV FP64 on a 1.5 GB variable-block matrix (>> 256 MB MALL)
@@ -350,6 +466,21 @@ def main():
                                        f"{B.nnz * 8 / 1e9:.2f} GB FP64 values",
                              "ms": ms, "gbs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS, "alg_bytes": bts}
         log(f"spmv large: {ms:.3f} ms, {gbs:.1f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% of HBM peak)")
+        if not args.no_cpu and not args.quick:
+            import oracle
+
+            if oracle.ref_available():
+                # the reference's BlockSpMat::mult on the same matrix and x, OpenMP threads
+                R = oracle.RefMat(oracle.Mat.of(B))
+                xs = x.cpu().numpy()
+                cpu_s = R.time_mult(xs, 5)
+                out["spmv_large"]["cpu_reference"] = {
+                    "ms": cpu_s * 1e3, "gbs": bts / cpu_s / 1e9, "gpu_speedup": cpu_s * 1e3 / ms,
+                    "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)),
+                    "kind": "reference", "sample": "vbcsr C++ core BlockSpMat::mult (oracle/_ref), mean of 5 calls "
+                                                   "after a warm-up"}
+                log(f"spmv large, reference CPU: {cpu_s * 1e3:.1f} ms")
+                del R
         # the same on the graphene matrix, FP64, K=1 (cache-resident)
         xg = torch.rand(G.n, dtype=torch.float64, device="cuda")
         yg = torch.zeros(G.n, dtype=torch.float64, device="cuda")
@@ -432,7 +563,9 @@ def main():
         log("CPU baseline (reference core on the host)")
         try:
             cpu, _ = cpu_baseline(G, K, M, seed)
-            log(f"cpu: {cpu['value']:.4g} moment-vecs/s ({cpu['kind']}, {cpu['cores']} threads)")
+            cpu["host"] = host_info()
+            log(f"cpu: {cpu['value']:.4g} moment-vecs/s ({cpu['kind']}, {cpu['cores']} threads, "
+                f"{cpu['host'].get('Model name', '?')})")
         except Exception as e:  # the baseline leg must not hide the GPU result
             cpu = {"value": None, "unit": "moment-vecs/s", "cores": None, "kind": "reference",
                    "sample": f"failed: {e}"}

@@ -304,16 +304,19 @@ def random_geometric_pairs(n: int, neighbours: float, rng: np.random.Generator):
 
 def mcweeny_start(nb: int = 5000, bs_lo: int = 4, bs_hi: int = 8, neighbours: float = 30.0,
                   seed: int = 0, h_scale: float = 0.1, h_shift: float = 0.5,
-                  normalize: bool = False) -> BlockCSR:
+                  normalize=False) -> BlockCSR:
     """McWeeny purification start P0 = h_scale * H + h_shift * I, with H a
     symmetric random-geometric Hamiltonian: blocks N(0,1) * exp(-r) (r in
     cutoff units, r = 0 on the diagonal).
 
     As written (normalize=False) the spectrum of P0 reaches well outside
     [-0.5, 1.5] at these sizes (about [-1.1, 2.1] at 600 block rows), where
-    P <- 3P^2 - 2P^3 diverges.  normalize=True first divides H by its
-    Gershgorin bound, so P0's spectrum lies in [0.5 - h_scale, 0.5 + h_scale]
-    and the iteration purifies."""
+    P <- 3P^2 - 2P^3 diverges.  normalize="spectral" first divides H by its
+    largest eigenvalue magnitude (Lanczos), so P0's spectrum spans
+    [0.5 - h_scale, 0.5 + h_scale] and the iteration purifies;
+    normalize=True divides by the Gershgorin bound instead (cheaper, but the
+    bound overestimates the spectrum several-fold, so P0 starts much closer
+    to 1/2 and purifies slowly)."""
     rng = np.random.default_rng(seed)
     bs = rng.integers(bs_lo, bs_hi + 1, size=nb).astype(np.int32)
     iu, ju, rr = random_geometric_pairs(nb, neighbours, rng)
@@ -322,7 +325,11 @@ def mcweeny_start(nb: int = 5000, bs_lo: int = 4, bs_hi: int = 8, neighbours: fl
         return g.standard_normal((n, r, c)) * np.exp(-rr[sel])[:, None, None]
 
     H = symmetric_from_upper(bs, iu, ju, draw, rng)
-    if normalize:
+    if normalize == "spectral":
+        lo, hi = spectral_bounds(H, tol=1e-8)
+        H = H.scaled(1.0 / max(abs(lo), abs(hi)))
+        H.meta.update(spectrum=(lo, hi))
+    elif normalize:
         H = gershgorin_scale(H, edge=1.0)
     P0 = H.scaled(h_scale, h_shift)
     P0.meta.update(name="mcweeny", nb=nb, seed=seed)
