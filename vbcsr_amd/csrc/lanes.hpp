@@ -117,17 +117,57 @@ __device__ __forceinline__ V quad_bcast(V v) {
     return out.v;
 }
 
+// Words (32-bit) of element w of a quad-contiguous X chunk set: chunk q of
+// the quad's 4*KT elements (16 bytes each) lives in lane q % 4 of the quad,
+// register q / 4.  Element f (0 <= f < 4*KT) of the quad's rows, broadcast
+// from its owner lane to the whole quad.
+template <typename T, int KT, int F, int NCL>
+__device__ __forceinline__ T quad_elem(const vbh_w4 (&xc)[NCL]) {
+    constexpr int WPE = int(sizeof(T) / 4);  // words per element
+    constexpr int word = F * WPE;
+    constexpr int q = word / 4, w = word % 4;
+    static_assert(q / 4 < NCL, "chunk out of range");
+    union {
+        T v;
+        int x[WPE];
+    } out;
+#pragma unroll
+    for (int i = 0; i < WPE; ++i) out.x[i] = quad_bcast<q % 4>(xc[q / 4][w + i]);
+    return out.v;
+}
+
+// acc[j] += sum_b ad[b] * x_b[j] for columns J..KT-1 of a quad-contiguous chunk set.
+template <typename T, typename A, int KT, int NCL, int J>
+__device__ __forceinline__ void quad_fma_cols(const A (&ad)[4], const vbh_w4 (&xc)[NCL], A (&acc)[KT]) {
+    if constexpr (J < KT) {
+        A t = Sc<A>::fma(ad[0], widen(quad_elem<T, KT, 0 * KT + J>(xc)), acc[J]);
+        t = Sc<A>::fma(ad[1], widen(quad_elem<T, KT, 1 * KT + J>(xc)), t);
+        t = Sc<A>::fma(ad[2], widen(quad_elem<T, KT, 2 * KT + J>(xc)), t);
+        acc[J] = Sc<A>::fma(ad[3], widen(quad_elem<T, KT, 3 * KT + J>(xc)), t);
+        quad_fma_cols<T, A, KT, NCL, J + 1>(ad, xc, acc);
+    }
+}
+
 // Slot loop of one lane for uniform 4x4 blocks.  The 4 rows of a block row
-// sit in one quad of lanes; lane a of the quad loads X row (4*col + a) of
-// every block (distinct addresses, no redundant traffic) and the quad shares
-// the 4 rows by DPP broadcast.  Column indices of 4*CB slots are preloaded
+// sit in one quad of lanes.  Column indices of 4*CB slots are preloaded
 // lane-parallel (lane a holds slots a, a+4, ...), and U slots are issued per
 // batch, so a row of 10 blocks costs 1 + 10/U memory latencies; U shrinks
 // with the slice width to keep 5+ waves per SIMD resident (measured: U = 2
-// beats 1 and 4 for 10 FP32 columns).
+// beats 1 and 4 for 10 FP32 columns).  Two ways to read a block's 4 X rows:
+//  * quad-contiguous (qx: one lane slice of all K columns, rows stored
+//    back to back, ldx == K, X 16-byte aligned -- the KPM recursion
+//    buffers): the 4 rows are 4*K contiguous elements, read as 16-byte
+//    chunks, chunk q by lane q % 4 of the quad, so each load instruction
+//    covers 64 contiguous bytes per quad (the texture addresser's cost goes
+//    with the cache lines an instruction touches);
+//  * per row (otherwise): lane a reads X row 4*col + a.
+// Either way each element reaches the whole quad by DPP broadcast, so X
+// traffic is not replicated across the block's rows.  Slots past the
+// partition's count are clamped to its last slot and enter with zero A
+// values (one select per A value instead of one per product).
 template <typename T, typename V, int KT, int VN>
 __device__ __forceinline__ void quad_slots(const ApplyArgs<T, V>& g, int kb, int cnt, int stride, int a, int v0,
-                                           int nvalid, typename Wide<T>::type (&acc)[KT]) {
+                                           int nvalid, bool qx, typename Wide<T>::type (&acc)[KT]) {
     using A = typename Wide<T>::type;
     constexpr int CB = 4;                                   // slots per lane per pass
     constexpr int per = int(KT * sizeof(T) / 4);            // VGPRs of one X slice
@@ -136,12 +176,14 @@ __device__ __forceinline__ void quad_slots(const ApplyArgs<T, V>& g, int kb, int
 #else
     constexpr int U = per <= 8 ? 4 : per <= 24 ? 2 : 1;     // slots per batch
 #endif
+    constexpr int NC = int(4 * KT * sizeof(T) / 16);        // 16-byte chunks of a quad's 4 X rows
+    constexpr int NCL = (NC + 3) / 4;                       // chunks per lane
+    static_assert(4 * KT * sizeof(T) % 16 == 0, "quad rows must fill whole 16-byte chunks");
     const DevPattern& p = g.p;
     // logical slot j of this lane's partition is physical slot kb + j*stride
     for (int j0 = 0; j0 < cnt; j0 += 4 * CB) {
         // ns >= 1; slot indices past the partition are clamped to its last
-        // slot, so every load below is unconditional and in bounds, and the
-        // extra slots' products are dropped by a select
+        // slot, so every load below is unconditional and in bounds
         const int ns = cnt - j0 < 4 * CB ? cnt - j0 : 4 * CB;
         const int k0 = kb + j0 * stride;
         int colv[CB];
@@ -150,37 +192,52 @@ __device__ __forceinline__ void quad_slots(const ApplyArgs<T, V>& g, int kb, int
 #pragma unroll
         for (int s0 = 0; s0 < 4 * CB; s0 += U) {
             if (s0 >= ns) break;
-            V av[U][4];
-            T xr[U][KT];
+            A ad[U][4];
+            int col[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const int s = s0 + u;
-                int col = 0;
                 switch (s & 3) {
-                    case 0: col = quad_bcast<0>(colv[s >> 2]); break;
-                    case 1: col = quad_bcast<1>(colv[s >> 2]); break;
-                    case 2: col = quad_bcast<2>(colv[s >> 2]); break;
-                    default: col = quad_bcast<3>(colv[s >> 2]); break;
+                    case 0: col[u] = quad_bcast<0>(colv[s >> 2]); break;
+                    case 1: col[u] = quad_bcast<1>(colv[s >> 2]); break;
+                    case 2: col[u] = quad_bcast<2>(colv[s >> 2]); break;
+                    default: col[u] = quad_bcast<3>(colv[s >> 2]); break;
                 }
                 const int sc = s < ns ? s : ns - 1;
                 const V* ar = static_cast<const V*>(__builtin_assume_aligned(
                     g.vals + int64_t(k0 + sc * stride) * 16 + a * 4, (4 * sizeof(V)) % 16 ? sizeof(V) : 16));
+                const bool on = s < ns;
 #pragma unroll
-                for (int b = 0; b < 4; ++b) av[u][b] = ar[b];
-                load_slice<T, KT, VN>(g.X + (int64_t(col) * 4 + a) * g.ldx + v0, nvalid, xr[u]);
+                for (int b = 0; b < 4; ++b) ad[u][b] = widen(pick(on, ar[b], Sc<V>::zero()));
             }
+            if (qx) {
+                vbh_w4 xc[U][NCL];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const bool on = s0 + u < ns;
+                for (int u = 0; u < U; ++u) {
+                    const vbh_w4* src = reinterpret_cast<const vbh_w4*>(g.X + int64_t(col[u]) * 4 * KT);
 #pragma unroll
-                for (int j = 0; j < KT; ++j) {
-                    const T x0 = quad_bcast<0>(xr[u][j]), x1 = quad_bcast<1>(xr[u][j]);
-                    const T x2 = quad_bcast<2>(xr[u][j]), x3 = quad_bcast<3>(xr[u][j]);
-                    A t = Sc<A>::fma(widen(av[u][0]), widen(x0), acc[j]);
-                    t = Sc<A>::fma(widen(av[u][1]), widen(x1), t);
-                    t = Sc<A>::fma(widen(av[u][2]), widen(x2), t);
-                    t = Sc<A>::fma(widen(av[u][3]), widen(x3), t);
-                    acc[j] = pick(on, t, acc[j]);
+                    for (int t = 0; t < NCL; ++t) xc[u][t] = src[a + 4 * t < NC ? a + 4 * t : NC - 1];
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    quad_fma_cols<T, A, KT, NCL, 0>(ad[u], xc[u], acc);
+                }
+            } else {
+                T xr[U][KT];
+#pragma unroll
+                for (int u = 0; u < U; ++u)
+                    load_slice<T, KT, VN>(g.X + (int64_t(col[u]) * 4 + a) * g.ldx + v0, nvalid, xr[u]);
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+#pragma unroll
+                    for (int j = 0; j < KT; ++j) {
+                        const T x0 = quad_bcast<0>(xr[u][j]), x1 = quad_bcast<1>(xr[u][j]);
+                        const T x2 = quad_bcast<2>(xr[u][j]), x3 = quad_bcast<3>(xr[u][j]);
+                        A t = Sc<A>::fma(ad[u][0], widen(x0), acc[j]);
+                        t = Sc<A>::fma(ad[u][1], widen(x1), t);
+                        t = Sc<A>::fma(ad[u][2], widen(x2), t);
+                        acc[j] = Sc<A>::fma(ad[u][3], widen(x3), t);
+                    }
                 }
             }
         }
@@ -235,7 +292,10 @@ __device__ __forceinline__ void lane_rows(const ApplyArgs<T, V>& g, int64_t sb, 
         for (int j = 0; j < KT; ++j) acc[j] = Sc<A>::zero();
         if constexpr (BS == 4) {
             const int kp = kb + part;
-            quad_slots<T, V, KT, VN>(g, kp, ke > kp ? (ke - kp + P - 1) / P : 0, P, a, v0, nvalid, acc);
+            // quad-contiguous X rows: one slice of all K columns, rows back to
+            // back, 16-byte aligned (wave-uniform)
+            const bool qx = S == 1 && nvalid == KT && g.ldx == KT && (reinterpret_cast<uintptr_t>(g.X) & 15) == 0;
+            quad_slots<T, V, KT, VN>(g, kp, ke > kp ? (ke - kp + P - 1) / P : 0, P, a, v0, nvalid, qx, acc);
         } else if constexpr (BS > 0) {
             for (int k = kb + part; k < ke; k += P) {
                 const int col = p.col_ind[k];
