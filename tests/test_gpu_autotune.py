@@ -1,5 +1,6 @@
 """GPU checks of the batched-KPM tile plans: every plan the autotuner can
-pick (matrix-core threshold x slot partitions) gives the oracle's moments
+pick (matrix-core threshold x lane-family slot partitions or row-wave rows
+per work-group, the latter given as negative parts) gives the oracle's moments
 (FP64: 1e-11 relative; precision "mixed": 1e-11 against the oracle on the
 FP32-rounded matrix), the autotuner reports a timed plan from its candidate
 set, and a second call on the same batch reuses that plan bit for bit."""
@@ -36,7 +37,7 @@ def kpm_api():
 
 
 @pytest.mark.parametrize("threshold", [4, 8, 17])
-@pytest.mark.parametrize("parts", [1, 2, 8])
+@pytest.mark.parametrize("parts", [1, 2, 8, -8, -32])
 def test_every_plan_matches_oracle(kpm_api, threshold, parts):
     K = kpm_api
     Hs, mats, V0s = batch(11)
@@ -59,7 +60,7 @@ def test_autotuner_picks_a_candidate_and_keeps_it(kpm_api):
     M = 24
     mu1 = K.kpm_moments_batched(mats, V0s, M, mode="two", precision="mixed")
     t, p, tuned = K.last_plan()
-    assert tuned and t in (4, 8, 17) and p >= 1
+    assert tuned and t in (4, 8, 17) and (p >= 1 or p in (-8, -16, -32))
     mu2 = K.kpm_moments_batched(mats, V0s, M, mode="two", precision="mixed")
     assert K.last_plan() == (t, p, True)
     assert np.array_equal(mu1, mu2)
@@ -122,3 +123,30 @@ def test_batch_with_only_class_rows_or_only_lane_rows(kpm_api, threshold):
     mu = K.kpm_moments_batched(mats[:1], V0s[:1], 16, mode="two")
     ref = O.kpm_moments(O.Mat.of(Hs[0]), np.asarray(V0s[0].to_numpy()), 16, mode=1)
     assert relerr(mu[0], ref) < 1e-11
+
+
+@pytest.mark.parametrize("K", [4, 16, 32, 64])
+@pytest.mark.parametrize("threshold", [4, 17])
+def test_row_wave_plans_wide(kpm_api, K, threshold):
+    """The row-wave kernel at the column counts it serves (FP64: K = 2 * 2^k,
+    FP32 recursion: K = 4 * 2^k), with rows of more than 64 blocks (LDS entry
+    lists refilled within a row) against the oracle."""
+    from vbcsr_amd import workloads as W
+
+    Kapi = kpm_api
+    Hs = W.batched_hamiltonians(count=3, nb_lo=60, nb_hi=200, neighbours=70.0, seed=41)
+    mats = [mk(A) for A in Hs]
+    V0s = []
+    for h, H in enumerate(mats):
+        v = H.create_multivector(K)
+        v.set_random(900 + h, normalize=False)
+        V0s.append(v)
+    Kapi.set_batch_plan(threshold, -16)
+    M = 12
+    for precision, tol in (("fp64", 1e-11), ("mixed", 1e-11), ("fp32", 2e-5)):
+        mu = Kapi.kpm_moments_batched(mats, V0s, M, mode="two", precision=precision)
+        assert Kapi.last_plan()[1] == -16, precision
+        for h in range(len(mats)):
+            A = Hs[h] if precision == "fp64" else rounded32(Hs[h])
+            ref = O.kpm_moments(O.Mat.of(A), np.asarray(V0s[h].to_numpy()), M, mode=1)
+            assert relerr(mu[h], ref) < tol, (precision, h)

@@ -430,9 +430,24 @@ RowConfig row_config(int K, int max_block, int uniform_bs);
 
 // Lane-per-row family (lanes.hpp): S lanes per scalar row, KT columns each.
 struct LaneShape {
-    int S, KT;  // S == 0: family not applicable
-    int P = 1;  // partitions of each row's slots (lane_parts)
+    int S, KT;   // S == 0: family not applicable
+    int P = 1;   // partitions of each row's slots (lane_parts)
+    int rw = 0;  // > 0: the row-wave kernel instead, rw scalar rows per work-group
 };
+// Columns per lane of the row-wave kernel (16 bytes of a row), or 0 when it
+// does not serve (T, K): real types, K a multiple of it with K / it a power
+// of two up to 64.
+template <typename T>
+inline int rowwave_cpl(int K) {
+    if constexpr (Sc<T>::is_complex) {
+        return 0;
+    } else {
+        const int cpl = int(16 / sizeof(T));
+        if (K < cpl || K % cpl) return 0;
+        const int lq = K / cpl;
+        return (lq & (lq - 1)) == 0 && lq <= 64 ? cpl : 0;
+    }
+}
 // Instantiated shapes: real K <= 16 one lane per row (KT in 4, 8, 10, 12, 16),
 // up to 64 with 2 or 4 lanes of 16 columns; complex KT = 8 with 1..8 lanes.
 // K == 1 runs here only for mixed value/vector types (the SpMV families are
@@ -561,6 +576,10 @@ template <typename T, typename V>
 void launch_batched_lanes(const BatchItem<T, V>* items, int nitems, int Gtot, int K, int ld, LaneShape sh, int step,
                           double alpha, double beta, double gamma0, double gamma, int mode, double* partials,
                           hipStream_t s);
+template <typename T, typename V>
+void launch_batched_rowwave(const BatchItem<T, V>* items, int nitems, int Gtot, int K, int ld, int rw, int step,
+                            double alpha, double beta, double gamma0, double gamma, int mode, double* partials,
+                            hipStream_t s);
 template <typename T, typename V>
 void launch_batched_mfma(const BatchItem<T, V>* items, int nitems, int Gtot, int K, int ld, int step, double alpha,
                          double beta, double gamma0, double gamma, int mode, double* partials, hipStream_t s);

@@ -417,7 +417,7 @@ static BatchLayout batch_layout(int count, vbh_matrix* const* ms, const vbh_matr
     L.seg.assign(count + 1, 0);
     L.seg2.assign(count + 1, 0);
     for (int h = 0; h < count; ++h) {
-        const int64_t rpg = sh.S ? lane_rows_per_group(sh.S, sh.P) : 1;
+        const int64_t rpg = sh.rw ? sh.rw : sh.S ? lane_rows_per_group(sh.S, sh.P) : 1;
         const bool split = mfma_ok && lists[h]->n_big > 0;
         const int64_t lrows = split ? lists[h]->n_small : ms[h]->n_rows;
         L.seg[h + 1] = L.seg[h] + (sh.S ? int((lrows + rpg - 1) / rpg) : ms[h]->G);
@@ -429,20 +429,23 @@ static BatchLayout batch_layout(int count, vbh_matrix* const* ms, const vbh_matr
 }
 
 // Tile plan of a batch: the matrix-core threshold (block rows of t..16 rows
-// on the matrix cores, t = 17: none) and the lane launch's slot partitions.
-// With autotuning on, the first call on a batch times one recursion step of
-// every candidate and keeps the fastest for later calls on the same batch.
+// on the matrix cores, t = 17: none) and the kernel of the other rows: the
+// lane family with P slot partitions, or the row-wave kernel with rw rows per
+// work-group.  With autotuning on, the first call on a batch times one
+// recursion step of every candidate and keeps the fastest for later calls on
+// the same batch.
 struct BatchPlan {
     int t = kMfmaMinRows;
     int P = 1;
     bool tuned = false;
+    int rw = 0;
 };
 // Plans per batch (process-wide, guarded by g_plan_mu), the switches, and the
 // plan of this thread's last batched call.
 static std::mutex g_plan_mu;
 static std::map<std::vector<uintptr_t>, BatchPlan> g_batch_plans;
 static int g_autotune = 1;
-static BatchPlan g_forced_plan{0, 1, false};  // t > 0: used instead of the cached or tuned plan
+static BatchPlan g_forced_plan{0, 1, false, 0};  // t > 0: used instead of the cached or tuned plan
 static thread_local BatchPlan g_last_plan;
 
 template <typename T, typename S, typename V>
@@ -462,6 +465,10 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
     LaneShape sh = lane_shape_of<T, V>(K);
     bool mfma_ok = !Sc<T>::is_complex && sh.S > 0;
     for (int h = 0; h < count; ++h) mfma_ok = mfma_ok && ms[h]->max_block <= 16;
+    // the row-wave kernel: real types, K = CPL * 2^k, block widths below 256
+    // and fewer than 2^24 columns (packed into its LDS entries)
+    bool rowwave_ok = sh.S > 0 && rowwave_cpl<T>(K) > 0;
+    for (int h = 0; h < count; ++h) rowwave_ok = rowwave_ok && ms[h]->n_cols < (int64_t(1) << 24);
     // row-class lists of every matrix for the plan's matrix-core threshold
     std::vector<const vbh_matrix::RowLists*> lists(count);
     auto select_all = [&](int t) {
@@ -497,6 +504,7 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
     if (forced) {
         plan = forced_plan;
         plan.P = std::min(plan.P, 64 / sh.S);
+        if (!rowwave_ok) plan.rw = 0;
     } else if (have_known) {
     } else if (sh.S) {
         select_all(kMfmaMinRows);
@@ -504,6 +512,7 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
     }
     select_all(plan.t);
     sh.P = sh.S ? plan.P : 1;
+    sh.rw = sh.S ? plan.rw : 0;
     BatchLayout lay = batch_layout(count, ms, lists.data(), sh, mfma_ok);
     std::vector<int>& seg = lay.seg;
     std::vector<int>& seg2 = lay.seg2;
@@ -570,8 +579,14 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
             launch_batched_mfma<T, V>(ditems, count, L.Gtot2, K, K, int(step), al, be, 0.0, -1.0, mode,
                                       pt + int64_t(L.Gtot1) * E, fork.side);
         }
-        if (L.Gtot1 > 0)
-            launch_batched_lanes<T, V>(ditems, count, L.Gtot1, K, K, shp, int(step), al, be, 0.0, -1.0, mode, pt, st);
+        if (L.Gtot1 > 0) {
+            if (shp.rw)
+                launch_batched_rowwave<T, V>(ditems, count, L.Gtot1, K, K, shp.rw, int(step), al, be, 0.0, -1.0, mode,
+                                             pt, st);
+            else
+                launch_batched_lanes<T, V>(ditems, count, L.Gtot1, K, K, shp, int(step), al, be, 0.0, -1.0, mode, pt,
+                                           st);
+        }
         if (L.Gtot2 > 0) {
             VBH_CHECK_HIP(hipEventRecord(fork.join, fork.side));
             VBH_CHECK_HIP(hipStreamWaitEvent(st, fork.join, 0));
@@ -596,10 +611,16 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
         for (int t : ts) {
             select_all(t);
             const int p0 = default_parts();
-            for (int P : {p0 / 2, p0, 2 * p0, 4 * p0}) {
-                if (P < 1 || P > pmax) continue;
+            // lane family: P around the row-count rule (rw = 0); row-wave
+            // kernel: rw rows per work-group (encoded as P = -rw)
+            std::vector<int> cands = {p0 / 2, p0, 2 * p0, 4 * p0};
+            if (rowwave_ok)
+                for (int rw : {8, 16, 32}) cands.push_back(-rw);
+            for (int P : cands) {
+                if (P == 0 || P > pmax || P < -64) continue;
                 LaneShape shc = sh;
-                shc.P = P;
+                shc.P = P > 0 ? P : 1;
+                shc.rw = P < 0 ? -P : 0;
                 BatchLayout Lc = batch_layout(count, ms, lists.data(), shc, mfma_ok);
                 fill_items(Lc);
                 Item* titems = (Item*)scratch(sizeof(Item) * count, 8);
@@ -615,7 +636,8 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
                 if (!plan.tuned || ms_ < best) {
                     best = ms_;
                     plan.t = t;
-                    plan.P = P;
+                    plan.P = shc.P;
+                    plan.rw = shc.rw;
                     plan.tuned = true;
                 }
             }
@@ -628,6 +650,7 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
         }
         select_all(plan.t);
         sh.P = plan.P;
+        sh.rw = plan.rw;
         lay = batch_layout(count, ms, lists.data(), sh, mfma_ok);
         Gtot1 = lay.Gtot1;
         Gtot2 = lay.Gtot2;
@@ -680,7 +703,7 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
             GraphCache& cache = sst.graph;
             std::vector<uintptr_t> key = {uintptr_t(ditems), uintptr_t(vecs), uintptr_t(partials), uintptr_t(count),
                                           uintptr_t(K), uintptr_t(nsteps), uintptr_t(mode), uintptr_t(sizeof(T)),
-                                          uintptr_t(Gtot), uintptr_t(Gtot1), uintptr_t(sh.P)};
+                                          uintptr_t(Gtot), uintptr_t(Gtot1), uintptr_t(sh.P), uintptr_t(sh.rw)};
             double ab[4] = {a1, b1, a2, b2};
             for (double v : ab) {
                 uint64_t bits;
@@ -938,12 +961,15 @@ VBH_API int vbh_set_autotune(int on) {
 
 VBH_API int vbh_set_batch_plan(int threshold, int parts) {
     return guard([&] {
-        VBH_REQUIRE(threshold == 0 || (threshold >= 1 && threshold <= 17 && parts >= 1 && parts <= 64 &&
-                                       (parts & (parts - 1)) == 0),
-                    "threshold must be 0 (release) or 1..17, with parts a power of two in 1..64");
+        const int mag = parts < 0 ? -parts : parts;
+        VBH_REQUIRE(threshold == 0 || (threshold >= 1 && threshold <= 17 && mag >= 1 && mag <= 64 &&
+                                       (mag & (mag - 1)) == 0),
+                    "threshold must be 0 (release) or 1..17, with parts a power of two in 1..64 (lane family) "
+                    "or its negative (row-wave kernel, rows per work-group)");
         std::lock_guard<std::mutex> lk(g_plan_mu);
         g_forced_plan.t = threshold;
-        g_forced_plan.P = parts;
+        g_forced_plan.P = parts > 0 ? parts : 1;
+        g_forced_plan.rw = parts < 0 ? -parts : 0;
         g_forced_plan.tuned = false;
     });
 }
@@ -952,7 +978,7 @@ VBH_API int vbh_kpm_last_plan(int* threshold, int* parts, int* tuned) {
     return guard([&] {
         VBH_REQUIRE(threshold && parts && tuned, "null argument");
         *threshold = g_last_plan.t;
-        *parts = g_last_plan.P;
+        *parts = g_last_plan.rw ? -g_last_plan.rw : g_last_plan.P;
         *tuned = g_last_plan.tuned ? 1 : 0;
     });
 }

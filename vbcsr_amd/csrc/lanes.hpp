@@ -512,6 +512,115 @@ __device__ __forceinline__ void store_n(T* dst, const T (&in)[N]) {
     __builtin_memcpy(dst, &v, sizeof(vec));
 }
 
+// ------------------------------------------------------------ row-wave family
+// Rows of the lane class of a batched step, one scalar row per wavefront at a
+// time.  The K columns split into LQ = K / CPL groups of CPL contiguous
+// columns (16 bytes), and the 64 lanes into LP = 64 / LQ partitions: lane
+// (partition p, group q) = p * LQ + q.  The row's blocks are flattened into
+// entries, one per stored column of the row (X row, value offset of the
+// block's first row at that column, block width), written to a per-wave LDS
+// list for up to kRowWaveEnt entries at a time; each step then handles LP
+// entries, one per partition, so a load instruction reads LP whole X rows
+// (each partition's lanes read one row, contiguous) -- the lane family reads
+// one slice per lane, and its load instructions touch one cache line per
+// lane.  Partitions merge by xor shuffles; the epilogue is the fused one.
+constexpr int kRowWaveEnt = 512;  // LDS entries per wave
+
+template <typename T, typename V, int CPL>
+__device__ __forceinline__ void rowwave_rows(const ApplyArgs<T, V>& g, int64_t sb, int64_t se, int lane, int wave,
+                                             int2* __restrict__ ent, double& d1, double& d2) {
+    using A = typename Wide<T>::type;
+    const DevPattern& p = g.p;
+    const int LQ = g.K / CPL;
+    const int lgq = __builtin_ctz(LQ);
+    const int LP = 64 >> lgq;
+    const int q = lane & (LQ - 1), part = lane >> lgq;
+    for (int64_t idx = sb + wave; idx < se; idx += kWaves) {
+        const int64_t srow = g.row_list ? int64_t(p.small_rows[idx]) : idx;
+        const int i = p.row_of[srow];
+        const int kb = __builtin_amdgcn_readfirstlane(p.row_ptr[i]);
+        const int ke = __builtin_amdgcn_readfirstlane(p.row_ptr[i + 1]);
+        const int r = p.rbs[i];
+        const int a = int(srow - p.roff[i]);
+        const int64_t vrow = p.voff[kb];
+        const int64_t xown = int64_t(p.coff[i]) + a;
+        A acc[CPL];
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) acc[j] = Sc<A>::zero();
+        for (int t0 = kb; t0 < ke; t0 += 64) {
+            const int nb = ke - t0 < 64 ? ke - t0 : 64;
+            const int2 mm = p.meta[t0 + (lane < nb ? lane : nb - 1)];
+            const int xo = mm.x, c = mm.y & 255, panel = mm.y >> 8;
+            for (int u0 = 0; u0 < nb;) {
+                // the run of slots from u0 whose entries fit the list
+                const int panel0 = __builtin_amdgcn_readlane(panel, u0);
+                const bool fits = lane >= u0 && lane < nb && panel + c - panel0 <= kRowWaveEnt;
+                const int u1 = u0 + __popcll(__ballot(fits));
+                const int W = __builtin_amdgcn_readlane(panel + c, u1 - 1) - panel0;
+                if (lane >= u0 && lane < u1) {
+                    for (int b = 0; b < c; ++b)
+                        ent[panel - panel0 + b] = make_int2(int(uint32_t(xo + b) | (uint32_t(c) << 24)), r * panel + b);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                // two steps per pass, all their loads issued first; entries
+                // past W repeat the last one with a zero value
+                for (int e0 = 0; e0 < W; e0 += 2 * LP) {
+                    A av[2];
+                    T xv[2][CPL];
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const int e = e0 + h * LP + part;
+                        const int2 en = ent[e < W ? e : W - 1];
+                        const int xrow = en.x & 0xffffff, cw = int(uint32_t(en.x) >> 24);
+                        const V vv = g.vals[vrow + en.y + a * cw];
+                        av[h] = e < W ? widen(vv) : Sc<A>::zero();
+                        load_n<T, CPL>(g.X + int64_t(xrow) * g.ldx + CPL * q, xv[h]);
+                    }
+#pragma unroll
+                    for (int h = 0; h < 2; ++h)
+#pragma unroll
+                        for (int j = 0; j < CPL; ++j) acc[j] = Sc<A>::fma(av[h], widen(xv[h][j]), acc[j]);
+                }
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                u0 = u1;
+            }
+        }
+        for (int m = LQ; m < 64; m <<= 1) {
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) acc[j] = acc[j] + shfl_xor(acc[j], m);
+        }
+        if (part == 0) emit_slice<T, V, CPL, CPL, true>(g, srow, xown, CPL * q, CPL, acc, d1, d2);
+    }
+}
+
+// Batched step over the lane-class rows with the row-wave kernel: rw rows
+// per work-group.
+template <typename T, typename V, int CPL>
+__global__ __launch_bounds__(kThreads) void k_batched_rowwave(const BatchItem<T, V>* __restrict__ items, int nitems,
+                                                              int Gtot, int K, int ld, int rw, int step, double alpha,
+                                                              double beta, double gamma0, double gamma, int mode,
+                                                              double* __restrict__ partials) {
+    __shared__ int2 ents[kWaves][kRowWaveEnt];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int gg = xcd_chunk(Gtot);
+    const int lo = batch_item<T, V, false>(items, nitems, gg);
+    using CItem = const __attribute__((address_space(4))) BatchItem<T, V>;
+    CItem& it = ((CItem*)items)[lo];
+    ApplyArgs<T, V> a = batch_args(items, lo, K, ld, step, alpha, beta, gamma0, gamma, mode);
+    a.ldw = ld;
+    a.row_list = it.row_list;
+    const int64_t sb = int64_t(gg - it.g_base) * rw;
+    const int64_t se = sb + rw < it.lane_rows ? sb + rw : it.lane_rows;
+    double d1 = 0.0, d2 = 0.0;
+    rowwave_rows<T, V, CPL>(a, sb, se, lane, wave, ents[wave], d1, d2);
+    write_partials(1, lane, wave, d1, d2, partials + int64_t(gg) * 2);
+}
+
 // Operands of slot u of the current 64-slot chunk, loaded straight into the
 // MFMA layout: a[m][e] = A[li][8m + 2lk + e], x[s][t] = X[xo + b(s, lk)][NT*li + t].
 // Indices are clamped into the block and into K, so every load is
@@ -809,6 +918,21 @@ void launch_batched_lanes(const BatchItem<T, V>* items, int nitems, int Gtot, in
 }
 
 template <typename T, typename V>
+void launch_batched_rowwave(const BatchItem<T, V>* items, int nitems, int Gtot, int K, int ld, int rw, int step,
+                            double alpha, double beta, double gamma0, double gamma, int mode, double* partials,
+                            hipStream_t s) {
+    if constexpr (Sc<T>::is_complex) {
+        throw Error(VBH_ERR_NOTSUP, "internal: the row-wave kernel is real-only");
+    } else {
+        constexpr int CPL = int(16 / sizeof(T));
+        VBH_REQUIRE(rowwave_cpl<T>(K) == CPL && ld % CPL == 0, "internal: row-wave kernel shape");
+        hipLaunchKernelGGL((k_batched_rowwave<T, V, CPL>), dim3(Gtot), dim3(kThreads), 0, s, items, nitems, Gtot, K,
+                           ld, rw, step, alpha, beta, gamma0, gamma, mode, partials);
+        VBH_CHECK_HIP(hipGetLastError());
+    }
+}
+
+template <typename T, typename V>
 void launch_batched_mfma(const BatchItem<T, V>* items, int nitems, int Gtot, int K, int ld, int step, double alpha,
                          double beta, double gamma0, double gamma, int mode, double* partials, hipStream_t s) {
     if constexpr (Sc<T>::is_complex) {
@@ -863,6 +987,8 @@ void launch_mfma(const ApplyArgs<T, V>& a, hipStream_t s) {
     template void launch_mfma<T, V>(const ApplyArgs<T, V>&, hipStream_t);                                       \
     template void launch_batched_mfma<T, V>(const BatchItem<T, V>*, int, int, int, int, int, double, double,    \
                                             double, double, int, double*, hipStream_t);                        \
+    template void launch_batched_rowwave<T, V>(const BatchItem<T, V>*, int, int, int, int, int, int, double,   \
+                                               double, double, double, int, double*, hipStream_t);             \
     template void launch_lanes<T, V>(const ApplyArgs<T, V>&, int64_t, LaneShape, int, hipStream_t);            \
     template void launch_batched_lanes<T, V>(const BatchItem<T, V>*, int, int, int, int, LaneShape, int, double, \
                                              double, double, double, int, double*, hipStream_t);
