@@ -678,6 +678,69 @@ __device__ __forceinline__ void mfma_slot(int c, int r, int li, int lk, int K, c
     }
 }
 
+// Epilogue of one matrix-core block row: Y rows row0..row0+r-1 from the
+// MFMA accumulators (register e of lane l holds row 4e + l/16 of column
+// NT*(l%16) + t of tile t), fused with beta*X_own + gamma*Z and the dots.
+template <typename T, typename V, int NT, bool VX, bool DOT>
+__device__ __forceinline__ void mfma_epilogue(const ApplyArgs<T, V>& g, const vbh_d4 (&acc)[NT], int r, int64_t row0,
+                                              int64_t x0r, int li, int lk, double& d1, double& d2) {
+    const int K = g.K;
+    // All epilogue loads (clamped rows and columns) are
+    // issued before the first use, so the row costs one more latency,
+    // not one per element
+    const bool need_x = DOT || g.beta != 0.0, need_z = g.gamma != 0.0, need_w = DOT && g.W != nullptr;
+    const int c0 = NT * li < K - NT ? NT * li : K - NT;
+    T xo[4][NT], zo[4][NT], wo[4][NT];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int a = 4 * e + lk;
+        const int ac = a < r ? a : r - 1;
+        const T* xp = g.X + (x0r + ac) * g.ldx;
+        const T* zp = g.Z + (row0 + ac) * g.ldz;
+        const T* wp = g.W + (row0 + ac) * g.ldw;
+        if constexpr (VX) {
+            if (need_x) load_n<T, NT>(xp + c0, xo[e]);
+            if (need_z) load_n<T, NT>(zp + c0, zo[e]);
+            if (need_w) load_n<T, NT>(wp + c0, wo[e]);
+        } else {
+#pragma unroll
+            for (int t = 0; t < NT; ++t) {
+                const int col = NT * li + t;
+                const int cc = col < K ? col : K - 1;
+                if (need_x) xo[e][t] = xp[cc];
+                if (need_z) zo[e][t] = zp[cc];
+                if (need_w) wo[e][t] = wp[cc];
+            }
+        }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int a = 4 * e + lk;
+        T ys[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t) {
+            const bool on = a < r && NT * li + t < K;
+            const double xv = need_x ? widen(xo[e][t]) : 0.0;
+            double y = g.alpha * acc[t][e];
+            if (g.beta != 0.0) y += g.beta * xv;
+            if (need_z) y += g.gamma * widen(zo[e][t]);
+            ys[t] = narrow_to<T>(y);
+            if constexpr (DOT) {
+                const double yw = widen(ys[t]);
+                d1 += on ? (need_w ? widen(wo[e][t]) : yw) * yw : 0.0;
+                d2 += on ? xv * yw : 0.0;
+            }
+        }
+        T* yp = g.Y + (row0 + a) * g.ldy;
+        if constexpr (VX) {
+            if (a < r && NT * li < K) store_n<T, NT>(yp + NT * li, ys);
+        } else {
+#pragma unroll
+            for (int t = 0; t < NT; ++t)
+                if (a < r && NT * li + t < K) yp[NT * li + t] = ys[t];
+        }
+    }}
+
 // Block rows [q0, q1) of the matrix-core list, wave-interleaved (row q0 +
 // wave, then + kWaves, ...), one block row at a time per wavefront.  Within
 // a row two slots are in flight: while slot u feeds the MFMAs, the operands
@@ -737,62 +800,7 @@ __device__ __forceinline__ void mfma_rows(const ApplyArgs<T, V>& g, int64_t q0, 
         }
         // the next row's first metadata chunk, in flight during the epilogue
         chunk_meta(__builtin_amdgcn_readfirstlane(n0.x), __builtin_amdgcn_readfirstlane(n0.y), mx, mw);
-        // register e of lane l holds row 4e + l/16; tile t holds column
-        // NT*(l%16) + t.  All epilogue loads (clamped rows and columns) are
-        // issued before the first use, so the row costs one more latency,
-        // not one per element
-        const bool need_x = DOT || g.beta != 0.0, need_z = g.gamma != 0.0, need_w = DOT && g.W != nullptr;
-        const int c0 = NT * li < K - NT ? NT * li : K - NT;
-        T xo[4][NT], zo[4][NT], wo[4][NT];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int a = 4 * e + lk;
-            const int ac = a < r ? a : r - 1;
-            const T* xp = g.X + (x0r + ac) * g.ldx;
-            const T* zp = g.Z + (row0 + ac) * g.ldz;
-            const T* wp = g.W + (row0 + ac) * g.ldw;
-            if constexpr (VX) {
-                if (need_x) load_n<T, NT>(xp + c0, xo[e]);
-                if (need_z) load_n<T, NT>(zp + c0, zo[e]);
-                if (need_w) load_n<T, NT>(wp + c0, wo[e]);
-            } else {
-#pragma unroll
-                for (int t = 0; t < NT; ++t) {
-                    const int col = NT * li + t;
-                    const int cc = col < K ? col : K - 1;
-                    if (need_x) xo[e][t] = xp[cc];
-                    if (need_z) zo[e][t] = zp[cc];
-                    if (need_w) wo[e][t] = wp[cc];
-                }
-            }
-        }
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int a = 4 * e + lk;
-            T ys[NT];
-#pragma unroll
-            for (int t = 0; t < NT; ++t) {
-                const bool on = a < r && NT * li + t < K;
-                const double xv = need_x ? widen(xo[e][t]) : 0.0;
-                double y = g.alpha * acc[t][e];
-                if (g.beta != 0.0) y += g.beta * xv;
-                if (need_z) y += g.gamma * widen(zo[e][t]);
-                ys[t] = narrow_to<T>(y);
-                if constexpr (DOT) {
-                    const double yw = widen(ys[t]);
-                    d1 += on ? (need_w ? widen(wo[e][t]) : yw) * yw : 0.0;
-                    d2 += on ? xv * yw : 0.0;
-                }
-            }
-            T* yp = g.Y + (row0 + a) * g.ldy;
-            if constexpr (VX) {
-                if (a < r && NT * li < K) store_n<T, NT>(yp + NT * li, ys);
-            } else {
-#pragma unroll
-                for (int t = 0; t < NT; ++t)
-                    if (a < r && NT * li + t < K) yp[NT * li + t] = ys[t];
-            }
-        }
+        mfma_epilogue<T, V, NT, VX, DOT>(g, acc, r, row0, x0r, li, lk, d1, d2);
         if (qn >= q1) break;
         q = qn;
         h0 = n0;
