@@ -159,3 +159,51 @@ def test_many_block_columns_numeric(numeric, tol):
     Cr = O.spgemm(m, m, 0.0)
     assert np.array_equal(C.row_ptr, Cr.row_ptr) and np.array_equal(C.col_ind, Cr.col_ind)
     assert relerr(C.get_values(), Cr.vals) < tol
+
+
+@pytest.mark.parametrize("numeric,tol", [("fp64", 1e-12), ("fp32", 1e-5)])
+def test_dense_panel_products(numeric, tol):
+    """Products whose C fills more than a fifth of its block pattern take the
+    dense-panel numeric kernel (B staged per inner block for a tile of rows,
+    skip rule per pair, block norms for the filter from the same registers).
+    Against the oracle: the pattern (after the threshold filter) and the
+    values, for P^2 and P^2 P of a purification start, at threshold 1e-5 and 0,
+    and the numeric pair count of the reference's rule."""
+    from vbcsr_amd import VBCSR
+    from vbcsr_amd import workloads as W
+
+    P0 = W.mcweeny_start(nb=150, neighbours=30.0, seed=4, normalize="spectral")
+    P = VBCSR.from_block_csr(P0.bs, P0.row_ptr, P0.col_ind, P0.vals)
+    pm = O.Mat.of(P0)
+    for thr in (1e-5, 0.0):
+        P2 = P.spmm(P, thr, numeric=numeric)
+        st = P.spmm_stats
+        assert st[2] >= 0.8 * P0.nb * P0.nb, st  # C filled enough for the panel kernel
+        p2 = O.spgemm(pm, pm, thr)
+        assert np.array_equal(P2.row_ptr, p2.row_ptr) and np.array_equal(P2.col_ind, p2.col_ind), thr
+        assert relerr(P2.get_values(), p2.vals) < tol, thr
+        P3 = P2.spmm(P, thr, numeric=numeric)
+        p3 = O.spgemm(p2, pm, thr)
+        assert np.array_equal(P3.row_ptr, p3.row_ptr) and np.array_equal(P3.col_ind, p3.col_ind), thr
+        assert relerr(P3.get_values(), p3.vals) < tol, thr
+    # numeric pairs: (i, k, j) with A_ik and B_kj stored, C_ij in the symbolic
+    # pattern (est_ij > thr) and ||A_ik|| ||B_kj|| >= thr / nnz_row(A_i)
+    # (ops/spmm/vbcsr.hpp:371-387), counted on the CPU from the block norms
+    nb, rp, thr = P0.nb, P0.row_ptr, 1e-5
+    nrm = np.zeros((nb, nb))
+    pres = np.zeros((nb, nb), bool)
+    bn = O.block_norms(pm)
+    for i in range(nb):
+        for k in range(rp[i], rp[i + 1]):
+            nrm[i, P0.col_ind[k]] = bn[k]
+            pres[i, P0.col_ind[k]] = True
+    sym = (nrm @ nrm) > thr
+    eps = thr / np.maximum(np.diff(rp), 1)
+    want = 0
+    for i in range(nb):
+        prod = nrm[i][:, None] * nrm  # [k, j]
+        ok = pres[i][:, None] & pres & sym[i][None, :] & ~(prod < eps[i])
+        want += int(ok.sum())
+    P.spmm(P, thr, numeric=numeric)
+    got = P.spmm_stats[1]
+    assert abs(got - want) <= 1e-3 * want, (got, want)

@@ -299,6 +299,9 @@ const vbh_matrix::RowLists& row_classes(vbh_matrix* m, int t, hipStream_t s);
 vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const int* row_ptr,
                          const int* col_ind, int dtype, hipStream_t s);
 void destroy_matrix(vbh_matrix* m);
+// Filtered copy of m: blocks with norm < thr dropped (norm = sqrt(d_norm[k])
+// when squared); null when every block is kept.  Device buffers only.
+vbh_matrix* filter_by_norms(vbh_matrix* m, double thr, const double* d_norm, bool squared, hipStream_t s);
 
 // Device buffers of one call, freed when it returns or throws.
 struct DevList {

@@ -534,6 +534,93 @@ static std::vector<int> diag_slots(const vbh_matrix* m, const int* diag_col) {
     return d;
 }
 
+// Filter compaction on the device: block k of row i is kept when its norm
+// is >= thr (norm = sqrt(norm2[k]) when squared); counts[i] = kept blocks of
+// row i.  One wavefront per row.
+__global__ __launch_bounds__(256) void k_keep_count(const int* __restrict__ row_ptr, int nbr,
+                                                    const double* __restrict__ norm, int squared, double thr,
+                                                    int* __restrict__ counts) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= nbr) return;
+    int n = 0;
+    for (int k = row_ptr[i] + lane; k < row_ptr[i + 1]; k += 64) {
+        const double v = squared ? sqrt(norm[k]) : norm[k];
+        n += v < thr ? 0 : 1;  // keep >= thr: block_csr.hpp:1411
+    }
+    for (int m = 32; m >= 1; m >>= 1) n += __shfl_xor(n, m, 64);
+    if (lane == 0) counts[i] = n;
+}
+
+// Kept blocks of every row, in order: new column indices and source slots.
+__global__ __launch_bounds__(256) void k_keep_compact(const int* __restrict__ row_ptr, const int* __restrict__ col_ind,
+                                                      int nbr, const double* __restrict__ norm, int squared, double thr,
+                                                      const int* __restrict__ new_ptr, int* __restrict__ new_ci,
+                                                      int* __restrict__ src) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= nbr) return;
+    int out = new_ptr[i];
+    for (int k0 = row_ptr[i]; k0 < row_ptr[i + 1]; k0 += 64) {
+        const int k = k0 + lane;
+        bool keep = false;
+        if (k < row_ptr[i + 1]) {
+            const double v = squared ? sqrt(norm[k]) : norm[k];
+            keep = !(v < thr);
+        }
+        const uint64_t mask = __ballot(keep);
+        if (keep) {
+            const int pos = out + __popcll(mask & ((uint64_t(1) << lane) - 1));
+            new_ci[pos] = col_ind[k];
+            src[pos] = k;
+        }
+        out += __popcll(mask);
+    }
+}
+
+// Filtered copy of m (blocks whose norm is below thr dropped), or null when
+// every block is kept.  The keep test and the compaction run on the device;
+// the host receives the per-row counts and the kept columns, which the
+// matrix handle mirrors.
+vbh_matrix* filter_by_norms(vbh_matrix* m, double thr, const double* d_norm, bool squared, hipStream_t s) {
+    if (m->nnzb == 0) return nullptr;
+    DevList tmp;
+    int* d_counts = tmp.own((int*)dalloc(sizeof(int) * size_t(m->nbr)));
+    const unsigned nb = unsigned((m->nbr + 3) / 4);
+    hipLaunchKernelGGL(k_keep_count, dim3(nb), dim3(256), 0, s, m->row_ptr, m->nbr, d_norm, squared ? 1 : 0, thr,
+                       d_counts);
+    VBH_CHECK_HIP(hipGetLastError());
+    std::vector<int> rp(m->nbr + 1, 0);
+    VBH_CHECK_HIP(hipMemcpyAsync(rp.data() + 1, d_counts, sizeof(int) * size_t(m->nbr), hipMemcpyDeviceToHost, s));
+    VBH_CHECK_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < m->nbr; ++i) rp[i + 1] += rp[i];
+    const int64_t total = rp[m->nbr];
+    if (total == m->nnzb) return nullptr;
+    std::vector<int> ci(total);
+    int* d_src = tmp.own((int*)dalloc(sizeof(int) * size_t(std::max<int64_t>(total, 1))));
+    if (total > 0) {
+        int* d_rp = tmp.own(upload(rp, s));
+        int* d_ci = tmp.own((int*)dalloc(sizeof(int) * size_t(total)));
+        hipLaunchKernelGGL(k_keep_compact, dim3(nb), dim3(256), 0, s, m->row_ptr, m->col_ind, m->nbr, d_norm,
+                           squared ? 1 : 0, thr, d_rp, d_ci, d_src);
+        VBH_CHECK_HIP(hipGetLastError());
+        VBH_CHECK_HIP(hipMemcpyAsync(ci.data(), d_ci, sizeof(int) * size_t(total), hipMemcpyDeviceToHost, s));
+        VBH_CHECK_HIP(hipStreamSynchronize(s));
+    }
+    MatrixGuard g(build_matrix(m->nbr, m->nbc, m->h_rbs.data(), m->h_cbs.data(), rp.data(), ci.data(), m->dtype, s));
+    vbh_matrix* c = g.m;
+    if (c->nnzb) {
+        dispatch_dtype(m->dtype, [&](auto tag) {
+            using T = decltype(tag);
+            hipLaunchKernelGGL(k_gather_blocks<T>, dim3(blocks_for(c->nnzb, 4)), dim3(256), 0, s,
+                               (const T*)m->vals, m->voff, d_src, (T*)c->vals, c->voff, c->nnzb);
+        });
+        VBH_CHECK_HIP(hipGetLastError());
+        VBH_CHECK_HIP(hipStreamSynchronize(s));
+    }
+    return g.release();
+}
+
 }  // namespace vbh
 
 using namespace vbh;
@@ -773,25 +860,6 @@ VBH_API int vbh_matrix_block_norms(vbh_matrix_t m, double* norms_dev, void* stre
     });
 }
 
-// New matrix with the given pattern whose blocks are gathered from m's slots.
-static vbh_matrix* gather_matrix(vbh_matrix* m, const std::vector<int>& rp, const std::vector<int>& ci,
-                                 const std::vector<int>& src_slot, hipStream_t s) {
-    MatrixGuard g(build_matrix(m->nbr, m->nbc, m->h_rbs.data(), m->h_cbs.data(), rp.data(), ci.data(), m->dtype, s));
-    vbh_matrix* c = g.m;
-    if (c->nnzb) {
-        DevList tmp;
-        int* d_src = tmp.own(upload(src_slot, s));
-        dispatch_dtype(m->dtype, [&](auto tag) {
-            using T = decltype(tag);
-            hipLaunchKernelGGL(k_gather_blocks<T>, dim3(blocks_for(c->nnzb, 4)), dim3(256), 0, s,
-                               (const T*)m->vals, m->voff, d_src, (T*)c->vals, c->voff, c->nnzb);
-        });
-        VBH_CHECK_HIP(hipGetLastError());
-        VBH_CHECK_HIP(hipStreamSynchronize(s));
-    }
-    return g.release();
-}
-
 VBH_API int vbh_matrix_filter(vbh_matrix_t m, double thr, vbh_matrix_t* out, void* stream) {
     return guard_on(stream, [&] {
         VBH_REQUIRE(m && out, "null argument");
@@ -805,22 +873,7 @@ VBH_API int vbh_matrix_filter(vbh_matrix_t m, double thr, vbh_matrix_t* out, voi
                                m->voff, m->nnzb, d_norms);
         });
         VBH_CHECK_HIP(hipGetLastError());
-        std::vector<double> norms(m->nnzb);
-        VBH_CHECK_HIP(hipMemcpyAsync(norms.data(), d_norms, sizeof(double) * m->nnzb, hipMemcpyDeviceToHost, s));
-        VBH_CHECK_HIP(hipStreamSynchronize(s));
-        std::vector<int> rp(m->nbr + 1, 0), ci, src;
-        ci.reserve(m->nnzb);
-        src.reserve(m->nnzb);
-        for (int i = 0; i < m->nbr; ++i) {
-            for (int k = m->h_row_ptr[i]; k < m->h_row_ptr[i + 1]; ++k) {
-                if (norms[k] < thr) continue;  // keep >= thr: block_csr.hpp:1411
-                ci.push_back(m->h_col_ind[k]);
-                src.push_back(k);
-            }
-            rp[i + 1] = int(ci.size());
-        }
-        if (int64_t(ci.size()) == m->nnzb) return;
-        *out = gather_matrix(m, rp, ci, src, s);
+        *out = filter_by_norms(m, thr, d_norms, false, s);
     });
 }
 

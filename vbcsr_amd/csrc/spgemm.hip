@@ -661,6 +661,276 @@ __global__ __launch_bounds__(256) void k_numeric_mfma_bf16_map(MfmaArgs a, int n
     }
 }
 
+// ------------------------------------------------------------ dense-panel numeric
+// Products whose C is dense enough (the late McWeeny iterations fill P to
+// most of its block pattern) are computed panel by panel from dense copies
+// of A and B (zeros where no block is stored).  A work-group owns
+// kPanelRows consecutive block rows of C, one per wave, and a window of
+// 64 * CPL scalar columns (CPL per lane).  For every inner block k it stages
+// the m_k rows of B over the window in LDS -- once for all its waves, so B
+// is read once per row tile rather than once per C row -- and each wave adds
+// A_ik (staged per wave) times that panel into registers.  A column takes a
+// product only when C's block (i, j) is in the symbolic pattern, B_kj is
+// stored, and the numeric rule keeps the pair (||A_ik|| ||B_kj|| >= thr /
+// nnz_row(A_i), ops/spmm/vbcsr.hpp:371-387); every C element receives its
+// products in k order and within a product in inner-index order, as in the
+// sparse kernels.  C is written once, and each block's squared Frobenius
+// norm is accumulated from the same registers for the filter.
+constexpr int kPanelWaves = 8;  // waves per work-group
+// block rows per wave: 2 for FP32 products, 1 for FP64 (two FP64 row sets
+// of accumulators and two staging sets exceed 128 VGPRs)
+template <typename CT>
+constexpr int panel_rows_per_wave() {
+    return sizeof(CT) == 4 ? 2 : 1;
+}
+constexpr double kPanelFill = 0.8;  // minimum block fill of C (measured: at 0.74 the sparse kernels are 7 % faster)
+
+template <typename CT>
+struct PanelArgs {
+    int nbr, nbk, nbc;
+    int64_t nca, ncb;          // scalar columns of A (= rows of B) and of B (= of C)
+    const CT* ad;              // dense A [n_rows(A)][nca]
+    const CT* bd;              // dense B [nca][ncb]
+    const double* anm;         // [nbr][nbk] ||A_ik|| or -1 (absent)
+    const double* bnm;         // [nbk][nbc] ||B_kj|| or -1
+    const int* cslot;          // [nbr][nbc] C's slot of (i, j) or -1
+    const int* colblk;         // [ncb] block column of each scalar column
+    const int* colb;           // [ncb] column within that block
+    const int* a_rbs;          // r_i
+    const int* a_roff;         // scalar row offset of block row i
+    const double* eps;         // [nbr] thr / nnz_row(A_i)
+    const int* k_size;         // [nbk] m_k
+    const int* k_off;          // [nbk] scalar offset of inner block k
+    const int* c_cbs;          // [nbc] c_j
+    const int64_t* c_voff;
+    double* c_vals;
+    double* c_norm2;           // [nnzb(C)] accumulated squared block norms
+    unsigned long long* npairs;
+    int n_tiles, n_groups;
+};
+
+// acc[a][t] += sum_q A[a][q] * B[q][col_t] over the m staged rows of B, all 8
+// accumulator rows (A rows past r are zero), columns masked by ok.
+template <typename CT, int CPL>
+__device__ __forceinline__ void panel_fma(int m, const CT* __restrict__ bt, const CT* __restrict__ as, int lane,
+                                          const bool (&ok)[CPL], CT (&acc)[8][CPL]) {
+    constexpr int W = 64 * CPL;
+#pragma unroll 1
+    for (int q = 0; q < m; ++q) {
+        CT bq[CPL];
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) bq[t] = ok[t] ? bt[q * W + lane * CPL + t] : CT(0);
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const CT av = as[a * m + q];
+#pragma unroll
+            for (int t = 0; t < CPL; ++t) acc[a][t] = Sc<CT>::fma(av, bq[t], acc[a][t]);
+        }
+    }
+}
+
+// At most 128 VGPRs: two work-groups per CU (four waves per SIMD) keep one
+// group's staging loads in flight while the other computes.
+template <typename CT, int CPL>
+__global__ __launch_bounds__(64 * kPanelWaves) __attribute__((amdgpu_waves_per_eu(4)))
+void k_numeric_panel(PanelArgs<CT> g) {
+    constexpr int W = 64 * CPL;                     // window columns
+    constexpr int NT = 64 * kPanelWaves;            // threads
+    constexpr int NE = 8 * W / NT;                  // staged B values per thread per k
+    constexpr int RW = panel_rows_per_wave<CT>();
+    constexpr int kPanelRows = kPanelWaves * RW;  // block rows per work-group
+    __shared__ CT bt[2][8 * W];
+    __shared__ double bn[2][W];
+    __shared__ CT as[kPanelWaves][2][RW][64];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    // the launch is rounded up to a multiple of 8 for the XCD remapping;
+    // groups past the panel grid exit (whole work-groups: no barrier is split)
+    const int gid = xcd_chunk(int(gridDim.x));
+    if (gid >= g.n_groups) return;
+    const int win = gid / g.n_tiles, tile = gid - win * g.n_tiles;
+    const int64_t c0 = int64_t(win) * W;
+    int row[RW], r[RW], cs[RW][CPL];
+    int64_t roff[RW];
+    double eps[RW];
+    bool row_ok[RW];
+    int bcol[CPL], cj[CPL];
+#pragma unroll
+    for (int t = 0; t < CPL; ++t) {
+        const int64_t col = c0 + lane * CPL + t;
+        const bool v = col < g.ncb;
+        cj[t] = v ? g.colblk[col] : 0;
+        bcol[t] = v ? g.colb[col] : 1;
+    }
+#pragma unroll
+    for (int h = 0; h < RW; ++h) {
+        const int i = tile * kPanelRows + wave * RW + h;
+        row[h] = i;
+        row_ok[h] = i < g.nbr;
+        r[h] = row_ok[h] ? g.a_rbs[i] : 0;
+        roff[h] = row_ok[h] ? g.a_roff[i] : 0;
+        eps[h] = row_ok[h] ? g.eps[i] : 0.0;
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            const bool v = c0 + lane * CPL + t < g.ncb;
+            cs[h][t] = (v && row_ok[h]) ? g.cslot[int64_t(i) * g.nbc + cj[t]] : -1;
+        }
+    }
+    // staging of inner block k: B rows k_off + q (q < m_k) over the window,
+    // ||B_kj|| per window column, and each wave's A_ik blocks.  Every load of
+    // a stage depends only on k (the dense copies hold zeros where no block
+    // is stored), and two stages are in flight while one is computed.
+    struct Stage {
+        CT sv[NE];
+        double snorm;
+        double an[RW];
+        CT av[RW];
+    };
+    const int jst = (tid < W && c0 + tid < g.ncb) ? g.colblk[c0 + tid] : -1;  // staged norm column's block
+    auto load_stage = [&](int k, Stage& st) {
+        const int m = g.k_size[k];
+        const int64_t ko = g.k_off[k];
+#pragma unroll
+        for (int u = 0; u < NE; ++u) {
+            const int e = tid + u * NT;
+            const int q = e / W, w = e - (e / W) * W;
+            const int64_t col = c0 + w;
+            const int qc = q < m ? q : m - 1;
+            const CT v = g.bd[(ko + qc) * g.ncb + (col < g.ncb ? col : g.ncb - 1)];
+            st.sv[u] = (q < m && col < g.ncb) ? v : CT(0);
+        }
+        st.snorm = jst >= 0 ? g.bnm[int64_t(k) * g.nbc + jst] : -1.0;
+        // A_ik element (a, q) = (lane / m, lane % m); rows past r read as zero
+        const int ar = lane / m, aq = lane - ar * m;
+#pragma unroll
+        for (int h = 0; h < RW; ++h) {
+            st.an[h] = row_ok[h] ? g.anm[int64_t(row[h]) * g.nbk + k] : -1.0;
+            const int arc = ar < r[h] ? ar : (r[h] > 0 ? r[h] - 1 : 0);
+            const CT v = g.ad[(roff[h] + arc) * g.nca + ko + (aq < m ? aq : m - 1)];
+            st.av[h] = ar < r[h] ? v : CT(0);
+        }
+    };
+    auto store_stage = [&](int buf, const Stage& st) {
+#pragma unroll
+        for (int u = 0; u < NE; ++u) bt[buf][tid + u * NT] = st.sv[u];
+        if (tid < W) bn[buf][tid] = st.snorm;
+#pragma unroll
+        for (int h = 0; h < RW; ++h) as[wave][buf][h][lane] = st.av[h];
+    };
+    CT acc[RW][8][CPL];
+#pragma unroll
+    for (int h = 0; h < RW; ++h)
+#pragma unroll
+        for (int a = 0; a < 8; ++a)
+#pragma unroll
+            for (int t = 0; t < CPL; ++t) acc[h][a][t] = CT(0);
+    unsigned long long used = 0;
+    auto compute = [&](int k, int buf, const double (&an)[RW]) {
+        const int m = g.k_size[k];
+#pragma unroll
+        for (int h = 0; h < RW; ++h) {
+            const double na = an[h];
+            if (!(row_ok[h] && na >= 0.0)) continue;
+            bool ok[CPL];
+#pragma unroll
+            for (int t = 0; t < CPL; ++t) {
+                const double nb = bn[buf][lane * CPL + t];
+                ok[t] = cs[h][t] >= 0 && nb >= 0.0 && !(na * nb < eps[h]);
+                used += (ok[t] && bcol[t] == 0) ? 1ull : 0ull;
+            }
+            panel_fma<CT, CPL>(m, bt[buf], as[wave][buf][h], lane, ok, acc[h]);
+        }
+    };
+    Stage s0, s1;
+    double an0[RW], an1[RW];
+    const int nk = g.nbk;
+    load_stage(0, s0);
+    if (nk > 1) load_stage(1, s1);
+    store_stage(0, s0);
+#pragma unroll
+    for (int h = 0; h < RW; ++h) an0[h] = s0.an[h];
+    __syncthreads();
+    for (int k = 0; k < nk; k += 2) {
+        // buffer 0 holds stage k; s1 (stage k+1) is in flight
+        if (k + 2 < nk) load_stage(k + 2, s0);
+        compute(k, 0, an0);
+        if (k + 1 >= nk) break;
+        store_stage(1, s1);
+#pragma unroll
+        for (int h = 0; h < RW; ++h) an1[h] = s1.an[h];
+        __syncthreads();
+        if (k + 3 < nk) load_stage(k + 3, s1);
+        compute(k + 1, 1, an1);
+        if (k + 2 >= nk) break;
+        store_stage(0, s0);
+#pragma unroll
+        for (int h = 0; h < RW; ++h) an0[h] = s0.an[h];
+        __syncthreads();
+    }
+    // C values once, and the blocks' squared norms
+#pragma unroll
+    for (int h = 0; h < RW; ++h) {
+#pragma unroll
+        for (int t = 0; t < CPL; ++t) {
+            if (cs[h][t] < 0) continue;
+            const int c = g.c_cbs[cj[t]];
+            double* cv = g.c_vals + g.c_voff[cs[h][t]] + bcol[t];
+            double n2 = 0.0;
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                if (a < r[h]) {
+                    const double v = double(acc[h][a][t]);
+                    cv[int64_t(a) * c] = v;
+                    n2 += v * v;
+                }
+            }
+            atomicAdd(&g.c_norm2[cs[h][t]], n2);
+        }
+    }
+    for (int s2 = 32; s2 >= 1; s2 >>= 1) used += __shfl_xor(used, s2, 64);
+    if (lane == 0 && used) atomicAdd(g.npairs, used);
+}
+
+// Dense copy of a block matrix: out[(roff_i + a) * ld + coff_j + b] = block
+// (i, j) element (a, b); one wavefront per stored block (out zeroed first).
+template <typename T, typename CT>
+__global__ __launch_bounds__(256) void k_densify(const T* __restrict__ vals, const int64_t* __restrict__ voff,
+                                                 const int* __restrict__ row_ptr, const int* __restrict__ col_ind,
+                                                 const int* __restrict__ roff, const int* __restrict__ coff,
+                                                 const int* __restrict__ cbs, int nbr, int64_t nnzb, int64_t ld,
+                                                 CT* __restrict__ out) {
+    const int64_t k = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (k >= nnzb) return;
+    int lo = 0, hi = nbr - 1;  // block row of slot k: last i with row_ptr[i] <= k
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (row_ptr[mid] <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    const int j = col_ind[k], c = cbs[j];
+    const int64_t n = voff[k + 1] - voff[k];
+    for (int64_t e = lane; e < n; e += 64) {
+        const int a = int(e / c), b = int(e - int64_t(a) * c);
+        out[(int64_t(roff[lo]) + a) * ld + coff[j] + b] = CT(vals[voff[k] + e]);
+    }
+}
+
+// map[i * ld + col_ind[k]] = val[k] (or k itself when val is null), for every
+// stored block k of row i.
+template <typename V>
+__global__ void k_block_map(const int* __restrict__ row_ptr, const int* __restrict__ col_ind, int nbr,
+                            const V* __restrict__ val, int64_t ld, V* __restrict__ map) {
+    const int i = blockIdx.x;
+    if (i >= nbr) return;
+    for (int k = row_ptr[i] + threadIdx.x; k < row_ptr[i + 1]; k += blockDim.x)
+        map[int64_t(i) * ld + col_ind[k]] = val ? val[k] : V(k);
+}
+
+__global__ void k_fill_f64(double* __restrict__ p, int64_t n, double v) {
+    for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += int64_t(gridDim.x) * blockDim.x)
+        p[e] = v;
+}
+
 static inline unsigned nblk4(int64_t n) { return unsigned((n + 3) / 4); }
 
 }  // namespace vbh
@@ -821,7 +1091,101 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
         vbh_matrix* Cm = C.m;
         VBH_CHECK_HIP(hipMemsetAsync(d_np, 0, sizeof(unsigned long long), s));
         // ---- numeric
-        if (Cm->nnzb > 0) {
+        double* d_cnorm2 = nullptr;  // squared C block norms from the panel kernel (fused filter)
+        const double fill = double(Cm->nnzb) / (double(std::max(nbr, 1)) * double(std::max(Cm->nbc, 1)));
+        bool panel = numeric != 2 && Cm->nnzb > 0 && A->dtype == VBH_F64 && A->max_block <= 8 &&
+                     B->max_block <= 8 && fill >= kPanelFill;
+        if (panel) {
+            // dense copies of A and B, block maps of norms and C slots: within
+            // half the free device memory, else the sparse kernels
+            const size_t ct = numeric == 1 ? sizeof(float) : sizeof(double);
+            const size_t need = ct * (size_t(A->n_rows) * size_t(A->n_cols) + size_t(B->n_rows) * size_t(B->n_cols)) +
+                                sizeof(double) * (size_t(nbr) * size_t(A->nbc) + size_t(B->nbr) * size_t(B->nbc)) +
+                                sizeof(int) * size_t(nbr) * size_t(Cm->nbc);
+            size_t free_b = 0, total_b = 0;
+            VBH_CHECK_HIP(hipMemGetInfo(&free_b, &total_b));
+            panel = need <= free_b / 2;
+        }
+        if (panel) {
+            d_cnorm2 = tmp.own((double*)dalloc(sizeof(double) * size_t(Cm->nnzb)));
+            VBH_CHECK_HIP(hipMemsetAsync(d_cnorm2, 0, sizeof(double) * size_t(Cm->nnzb), s));
+            auto go = [&](auto* ctag) {
+                using CT = std::remove_pointer_t<decltype(ctag)>;
+                constexpr int CPL = 2;
+                DevList pt;  // the panel's temporaries
+                CT* ad = pt.own((CT*)dalloc(sizeof(CT) * size_t(A->n_rows) * size_t(A->n_cols)));
+                CT* bd = pt.own((CT*)dalloc(sizeof(CT) * size_t(B->n_rows) * size_t(B->n_cols)));
+                VBH_CHECK_HIP(hipMemsetAsync(ad, 0, sizeof(CT) * size_t(A->n_rows) * size_t(A->n_cols), s));
+                VBH_CHECK_HIP(hipMemsetAsync(bd, 0, sizeof(CT) * size_t(B->n_rows) * size_t(B->n_cols), s));
+                if (A->nnzb)
+                    hipLaunchKernelGGL((k_densify<double, CT>), dim3(nblk4(A->nnzb)), dim3(256), 0, s,
+                                       (const double*)A->vals, A->voff, A->row_ptr, A->col_ind, A->roff, A->coff,
+                                       A->cbs, A->nbr, A->nnzb, int64_t(A->n_cols), ad);
+                if (B->nnzb)
+                    hipLaunchKernelGGL((k_densify<double, CT>), dim3(nblk4(B->nnzb)), dim3(256), 0, s,
+                                       (const double*)B->vals, B->voff, B->row_ptr, B->col_ind, B->roff, B->coff,
+                                       B->cbs, B->nbr, B->nnzb, int64_t(B->n_cols), bd);
+                VBH_CHECK_HIP(hipGetLastError());
+                double* anm = pt.own((double*)dalloc(sizeof(double) * size_t(nbr) * size_t(A->nbc)));
+                double* bnm = pt.own((double*)dalloc(sizeof(double) * size_t(B->nbr) * size_t(B->nbc)));
+                int* csl = pt.own((int*)dalloc(sizeof(int) * size_t(nbr) * size_t(Cm->nbc)));
+                hipLaunchKernelGGL(k_fill_f64, dim3(2048), dim3(256), 0, s, anm, int64_t(nbr) * A->nbc, -1.0);
+                hipLaunchKernelGGL(k_fill_f64, dim3(2048), dim3(256), 0, s, bnm, int64_t(B->nbr) * B->nbc, -1.0);
+                VBH_CHECK_HIP(hipMemsetAsync(csl, 0xff, sizeof(int) * size_t(nbr) * size_t(Cm->nbc), s));
+                hipLaunchKernelGGL(k_block_map<double>, dim3(nbr), dim3(256), 0, s, A->row_ptr, A->col_ind, nbr,
+                                   (const double*)an, int64_t(A->nbc), anm);
+                hipLaunchKernelGGL(k_block_map<double>, dim3(B->nbr), dim3(256), 0, s, B->row_ptr, B->col_ind, B->nbr,
+                                   (const double*)bn, int64_t(B->nbc), bnm);
+                hipLaunchKernelGGL(k_block_map<int>, dim3(nbr), dim3(256), 0, s, Cm->row_ptr, Cm->col_ind, nbr,
+                                   (const int*)nullptr, int64_t(Cm->nbc), csl);
+                VBH_CHECK_HIP(hipGetLastError());
+                std::vector<int> colblk(size_t(B->n_cols)), colb(size_t(B->n_cols));
+                for (int j = 0; j < B->nbc; ++j)
+                    for (int b = 0; b < B->h_cbs[j]; ++b) {
+                        colblk[size_t(B->h_coff[j] + b)] = j;
+                        colb[size_t(B->h_coff[j] + b)] = b;
+                    }
+                std::vector<double> eps(nbr);
+                for (int i = 0; i < nbr; ++i) {
+                    const int cnt = A->h_row_ptr[i + 1] - A->h_row_ptr[i];
+                    eps[i] = thr / double(cnt > 1 ? cnt : 1);
+                }
+                PanelArgs<CT> pa{};
+                pa.nbr = nbr;
+                pa.nbk = A->nbc;
+                pa.nbc = Cm->nbc;
+                pa.nca = A->n_cols;
+                pa.ncb = B->n_cols;
+                pa.ad = ad;
+                pa.bd = bd;
+                pa.anm = anm;
+                pa.bnm = bnm;
+                pa.cslot = csl;
+                pa.colblk = pt.own(upload(colblk, s));
+                pa.colb = pt.own(upload(colb, s));
+                pa.a_rbs = A->rbs;
+                pa.a_roff = A->roff;
+                pa.eps = pt.own(upload(eps, s));
+                pa.k_size = B->rbs;
+                pa.k_off = B->roff;
+                pa.c_cbs = Cm->cbs;
+                pa.c_voff = Cm->voff;
+                pa.c_vals = (double*)Cm->vals;
+                pa.c_norm2 = d_cnorm2;
+                pa.npairs = d_np;
+                constexpr int rows_per_group = kPanelWaves * panel_rows_per_wave<CT>();
+                pa.n_tiles = (nbr + rows_per_group - 1) / rows_per_group;
+                const int64_t nwin = (B->n_cols + 64 * CPL - 1) / (64 * CPL);
+                VBH_REQUIRE(int64_t(pa.n_tiles) * nwin <= INT32_MAX - 8, "internal: panel grid too large");
+                pa.n_groups = int(pa.n_tiles * nwin);
+                hipLaunchKernelGGL((k_numeric_panel<CT, CPL>), dim3((pa.n_groups + 7) & ~7), dim3(64 * kPanelWaves), 0,
+                                   s, pa);
+                VBH_CHECK_HIP(hipGetLastError());
+                VBH_CHECK_HIP(hipStreamSynchronize(s));  // before the temporaries are freed
+            };
+            if (numeric == 1) go((float*)nullptr);
+            else go((double*)nullptr);
+        } else if (Cm->nnzb > 0) {
             if (numeric == 2) {
                 std::vector<int> colstart(Cm->nnzb), width(nbr);
                 for (int i = 0; i < nbr; ++i) {
@@ -898,8 +1262,13 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
         // ---- filter
         vbh_matrix* F = nullptr;
         if (thr > 0.0 && C.m->nnzb > 0) {
-            const int rc = vbh_matrix_filter(C.m, thr, &F, stream);
-            if (rc != VBH_OK) throw Error(rc, vbh_last_error());
+            if (d_cnorm2) {
+                // norms from the numeric phase's registers: no separate pass
+                F = filter_by_norms(C.m, thr, d_cnorm2, true, s);
+            } else {
+                const int rc = vbh_matrix_filter(C.m, thr, &F, stream);
+                if (rc != VBH_OK) throw Error(rc, vbh_last_error());
+            }
         }
         if (F) C.reset(F);
         if (stats) {
