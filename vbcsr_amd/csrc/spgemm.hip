@@ -683,6 +683,7 @@ template <typename CT>
 constexpr int panel_rows_per_wave() {
     return sizeof(CT) == 4 ? 2 : 1;
 }
+constexpr double kDense16Fill = 0.1;  // minimum block fill of C for the dense BF16 products
 constexpr double kPanelFill = 0.8;  // minimum block fill of C (measured: at 0.74 the sparse kernels are 7 % faster)
 
 template <typename CT>
@@ -890,14 +891,23 @@ void k_numeric_panel(PanelArgs<CT> g) {
     if (lane == 0 && used) atomicAdd(g.npairs, used);
 }
 
+// Storage conversions of the dense copies: FP64, FP32, or BF16 bits
+// (round-to-nearest-even of the FP32 value).
+template <typename CT>
+__device__ inline CT to_dense(double x) {
+    if constexpr (std::is_same_v<CT, short>) return to_bf16(x);
+    else return CT(x);
+}
+
 // Dense copy of a block matrix: out[(roff_i + a) * ld + coff_j + b] = block
-// (i, j) element (a, b); one wavefront per stored block (out zeroed first).
+// (i, j) element (a, b), or out[(coff_j + b) * ld + roff_i + a] when
+// transposed; one wavefront per stored block (out zeroed first).
 template <typename T, typename CT>
 __global__ __launch_bounds__(256) void k_densify(const T* __restrict__ vals, const int64_t* __restrict__ voff,
                                                  const int* __restrict__ row_ptr, const int* __restrict__ col_ind,
                                                  const int* __restrict__ roff, const int* __restrict__ coff,
                                                  const int* __restrict__ cbs, int nbr, int64_t nnzb, int64_t ld,
-                                                 CT* __restrict__ out) {
+                                                 int transposed, CT* __restrict__ out) {
     const int64_t k = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (k >= nnzb) return;
@@ -911,8 +921,127 @@ __global__ __launch_bounds__(256) void k_densify(const T* __restrict__ vals, con
     const int64_t n = voff[k + 1] - voff[k];
     for (int64_t e = lane; e < n; e += 64) {
         const int a = int(e / c), b = int(e - int64_t(a) * c);
-        out[(int64_t(roff[lo]) + a) * ld + coff[j] + b] = CT(vals[voff[k] + e]);
+        const int64_t row = int64_t(roff[lo]) + a, col = int64_t(coff[j]) + b;
+        out[transposed ? col * ld + row : row * ld + col] = to_dense<CT>(double(vals[voff[k] + e]));
     }
+}
+
+// ------------------------------------------------------------ BF16 dense products
+// C (FP32, [Mp][Np]) = A (BF16, [Mp][Kp]) times B^T (BF16, [Np][Kp]): both
+// operands k-contiguous, every extent padded with zeros to the tile sizes, so
+// the kernel has no bounds checks.  A work-group computes a 128 x 128 tile of
+// C, each of its 4 waves a 64 x 64 quarter as 4 x 4 tiles of
+// v_mfma_f32_16x16x32_bf16 (FP32 accumulation).  Per k-step of 32 the
+// group stages A's and B's 128 x 32 slices in LDS (row pitch 40 elements so
+// the 16 rows an operand read touches fall on different banks); the next
+// step's slices are loaded into registers while the current one is
+// multiplied.
+constexpr int kGemmTile = 128;
+constexpr int kGemmK = 32;
+constexpr int kGemmPitch = kGemmK + 8;
+
+__global__ __launch_bounds__(256) void k_gemm_bf16_nt(const short* __restrict__ A, const short* __restrict__ BT,
+                                                      float* __restrict__ C, int64_t Kp, int64_t ldc, int tiles_n,
+                                                      int n_tiles) {
+    __shared__ __attribute__((aligned(16))) short as[2][kGemmTile * kGemmPitch];
+    __shared__ __attribute__((aligned(16))) short bs[2][kGemmTile * kGemmPitch];
+    // the launch is rounded up to a multiple of 8 for the XCD remapping
+    const int gid = xcd_chunk(int(gridDim.x));
+    if (gid >= n_tiles) return;
+    const int tm = gid / tiles_n, tn = gid - tm * tiles_n;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int64_t m0 = int64_t(tm) * kGemmTile, n0 = int64_t(tn) * kGemmTile;
+    // staging: 512 16-byte chunks per operand slice, two per thread
+    // (chunk c: row c / 4, k-offset 8 * (c % 4))
+    typedef __attribute__((__vector_size__(4 * sizeof(int)))) int i32x4;
+    i32x4 ra[2], rb[2];
+    auto load = [&](int64_t k0) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int c = tid + 256 * u;
+            const int row = c >> 2, kc = (c & 3) * 8;
+            ra[u] = *reinterpret_cast<const i32x4*>(A + (m0 + row) * Kp + k0 + kc);
+            rb[u] = *reinterpret_cast<const i32x4*>(BT + (n0 + row) * Kp + k0 + kc);
+        }
+    };
+    auto store = [&](int buf) {
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int c = tid + 256 * u;
+            const int row = c >> 2, kc = (c & 3) * 8;
+            *reinterpret_cast<i32x4*>(&as[buf][row * kGemmPitch + kc]) = ra[u];
+            *reinterpret_cast<i32x4*>(&bs[buf][row * kGemmPitch + kc]) = rb[u];
+        }
+    };
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    const int lr = lane & 15, lk = (lane >> 4) * 8;
+    const int nsteps = int(Kp / kGemmK);
+    load(0);
+    store(0);
+    __syncthreads();
+    for (int st = 0; st < nsteps; ++st) {
+        const int buf = st & 1;
+        if (st + 1 < nsteps) load(int64_t(st + 1) * kGemmK);
+        bf16x8 av[4], bv[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            av[i] = *reinterpret_cast<const bf16x8*>(&as[buf][(wm * 64 + i * 16 + lr) * kGemmPitch + lk]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            bv[j] = *reinterpret_cast<const bf16x8*>(&bs[buf][(wn * 64 + j * 16 + lr) * kGemmPitch + lk]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+        if (st + 1 < nsteps) store(buf ^ 1);
+        __syncthreads();
+    }
+    // result lane l: column l % 16, rows 4 (l / 16) .. +3 of each 16 x 16 tile
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int64_t row = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + e;
+                const int64_t col = n0 + wn * 64 + j * 16 + lr;
+                C[row * ldc + col] = acc[i][j][e];
+            }
+}
+
+// C's blocks (its symbolic pattern) from the dense FP32 product, widened to
+// FP64, with each block's squared Frobenius norm; one wavefront per block.
+__global__ __launch_bounds__(256) void k_scatter_dense(const float* __restrict__ Cd, int64_t ldc,
+                                                       const int* __restrict__ row_ptr, const int* __restrict__ col_ind,
+                                                       const int64_t* __restrict__ voff, const int* __restrict__ roff,
+                                                       const int* __restrict__ coff, const int* __restrict__ cbs,
+                                                       int nbr, int64_t nnzb, double* __restrict__ vals,
+                                                       double* __restrict__ norm2) {
+    const int64_t k = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (k >= nnzb) return;
+    int lo = 0, hi = nbr - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (row_ptr[mid] <= k) lo = mid;
+        else hi = mid - 1;
+    }
+    const int j = col_ind[k], c = cbs[j];
+    const int64_t n = voff[k + 1] - voff[k];
+    double s = 0.0;
+    for (int64_t e = lane; e < n; e += 64) {
+        const int a = int(e / c), b = int(e - int64_t(a) * c);
+        const double v = double(Cd[(int64_t(roff[lo]) + a) * ldc + coff[j] + b]);
+        vals[voff[k] + e] = v;
+        s += v * v;
+    }
+    for (int m = 32; m >= 1; m >>= 1) s += __shfl_xor(s, m, 64);
+    if (lane == 0) norm2[k] = s;
 }
 
 // map[i * ld + col_ind[k]] = val[k] (or k itself when val is null), for every
@@ -1106,7 +1235,50 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
             VBH_CHECK_HIP(hipMemGetInfo(&free_b, &total_b));
             panel = need <= free_b / 2;
         }
-        if (panel) {
+        // BF16 products of dense-enough results: one dense BF16 matrix-core
+        // GEMM (FP32 accumulation) over zero-padded dense copies, then C's
+        // pattern blocks and their norms.  Every stored pair is multiplied:
+        // the numeric skip rule drops products of norm below thr / nnz_row,
+        // under BF16 rounding.
+        const int64_t Mp = (A->n_rows + kGemmTile - 1) / kGemmTile * kGemmTile;
+        const int64_t Np = (B->n_cols + kGemmTile - 1) / kGemmTile * kGemmTile;
+        const int64_t Kp = (A->n_cols + kGemmK - 1) / kGemmK * kGemmK;
+        bool dense16 = numeric == 2 && Cm->nnzb > 0 && A->dtype == VBH_F64 && fill >= kDense16Fill &&
+                       Mp / kGemmTile * (Np / kGemmTile) <= INT32_MAX - 8;
+        if (dense16) {
+            const size_t need = size_t(2) * size_t(Mp + Np) * size_t(Kp) + size_t(4) * size_t(Mp) * size_t(Np);
+            size_t free_b = 0, total_b = 0;
+            VBH_CHECK_HIP(hipMemGetInfo(&free_b, &total_b));
+            dense16 = need <= free_b / 2;
+        }
+        if (dense16) {
+            d_cnorm2 = tmp.own((double*)dalloc(sizeof(double) * size_t(Cm->nnzb)));
+            DevList dt;
+            short* a16 = dt.own((short*)dalloc(sizeof(short) * size_t(Mp) * size_t(Kp)));
+            short* b16 = dt.own((short*)dalloc(sizeof(short) * size_t(Np) * size_t(Kp)));
+            float* cd = dt.own((float*)dalloc(sizeof(float) * size_t(Mp) * size_t(Np)));
+            VBH_CHECK_HIP(hipMemsetAsync(a16, 0, sizeof(short) * size_t(Mp) * size_t(Kp), s));
+            VBH_CHECK_HIP(hipMemsetAsync(b16, 0, sizeof(short) * size_t(Np) * size_t(Kp), s));
+            if (A->nnzb)
+                hipLaunchKernelGGL((k_densify<double, short>), dim3(nblk4(A->nnzb)), dim3(256), 0, s,
+                                   (const double*)A->vals, A->voff, A->row_ptr, A->col_ind, A->roff, A->coff, A->cbs,
+                                   A->nbr, A->nnzb, Kp, 0, a16);
+            if (B->nnzb)
+                hipLaunchKernelGGL((k_densify<double, short>), dim3(nblk4(B->nnzb)), dim3(256), 0, s,
+                                   (const double*)B->vals, B->voff, B->row_ptr, B->col_ind, B->roff, B->coff, B->cbs,
+                                   B->nbr, B->nnzb, Kp, 1, b16);
+            VBH_CHECK_HIP(hipGetLastError());
+            const int tiles_n = int(Np / kGemmTile);
+            const int n_tiles = int(Mp / kGemmTile) * tiles_n;
+            hipLaunchKernelGGL(k_gemm_bf16_nt, dim3((n_tiles + 7) & ~7), dim3(256), 0, s, a16, b16, cd, Kp, Np,
+                               tiles_n, n_tiles);
+            VBH_CHECK_HIP(hipGetLastError());
+            hipLaunchKernelGGL(k_scatter_dense, dim3(nblk4(Cm->nnzb)), dim3(256), 0, s, cd, Np, Cm->row_ptr,
+                               Cm->col_ind, Cm->voff, Cm->roff, Cm->coff, Cm->cbs, nbr, Cm->nnzb, (double*)Cm->vals,
+                               d_cnorm2);
+            VBH_CHECK_HIP(hipGetLastError());
+            VBH_CHECK_HIP(hipStreamSynchronize(s));  // before the temporaries are freed
+        } else if (panel) {
             d_cnorm2 = tmp.own((double*)dalloc(sizeof(double) * size_t(Cm->nnzb)));
             VBH_CHECK_HIP(hipMemsetAsync(d_cnorm2, 0, sizeof(double) * size_t(Cm->nnzb), s));
             auto go = [&](auto* ctag) {
@@ -1120,11 +1292,11 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
                 if (A->nnzb)
                     hipLaunchKernelGGL((k_densify<double, CT>), dim3(nblk4(A->nnzb)), dim3(256), 0, s,
                                        (const double*)A->vals, A->voff, A->row_ptr, A->col_ind, A->roff, A->coff,
-                                       A->cbs, A->nbr, A->nnzb, int64_t(A->n_cols), ad);
+                                       A->cbs, A->nbr, A->nnzb, int64_t(A->n_cols), 0, ad);
                 if (B->nnzb)
                     hipLaunchKernelGGL((k_densify<double, CT>), dim3(nblk4(B->nnzb)), dim3(256), 0, s,
                                        (const double*)B->vals, B->voff, B->row_ptr, B->col_ind, B->roff, B->coff,
-                                       B->cbs, B->nbr, B->nnzb, int64_t(B->n_cols), bd);
+                                       B->cbs, B->nbr, B->nnzb, int64_t(B->n_cols), 0, bd);
                 VBH_CHECK_HIP(hipGetLastError());
                 double* anm = pt.own((double*)dalloc(sizeof(double) * size_t(nbr) * size_t(A->nbc)));
                 double* bnm = pt.own((double*)dalloc(sizeof(double) * size_t(B->nbr) * size_t(B->nbc)));
@@ -1275,7 +1447,7 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
             int64_t sp = 0;
             for (int64_t p : pairs) sp += p;
             stats[0] = sp;
-            stats[1] = int64_t(np);
+            stats[1] = dense16 ? sp : int64_t(np);  // dense BF16 products multiply every candidate pair
             stats[2] = before;
             stats[3] = C.m->nnzb;
         }
