@@ -184,6 +184,86 @@ __global__ __launch_bounds__(256) void k_symbolic_dense(SymArgs a, int nbc) {
     if (tid == 255) a.out_count[i] = scan[255];
 }
 
+// Dense symbolic phase for products whose candidate pattern is dense: the
+// estimates est_ij = sum_k ||A_ik|| ||B_kj|| (common.hpp:620-664) of all rows
+// at once as one FP64 product of the block-norm matrices (zeros where no block
+// is stored; the padded extents are multiples of the tile sizes).  A
+// work-group computes a 64 x 64 tile, each thread a 4 x 4 micro-tile, over
+// k-steps of 16 staged in LDS.
+constexpr int kEstTile = 64;
+constexpr int kEstK = 16;
+
+__global__ __launch_bounds__(256) void k_gemm_f64(const double* __restrict__ A, const double* __restrict__ B,
+                                                  double* __restrict__ C, int64_t Kp, int64_t Np, int tiles_n,
+                                                  int n_tiles) {
+    __shared__ double as[kEstK][kEstTile + 1];
+    __shared__ double bs[kEstK][kEstTile + 1];
+    const int gid = xcd_chunk(int(gridDim.x));
+    if (gid >= n_tiles) return;
+    const int tm = gid / tiles_n, tn = gid - tm * tiles_n;
+    const int64_t m0 = int64_t(tm) * kEstTile, n0 = int64_t(tn) * kEstTile;
+    const int tid = threadIdx.x, ty = tid >> 4, tx = tid & 15;
+    double acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+    for (int64_t k0 = 0; k0 < Kp; k0 += kEstK) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u;
+            as[e & 15][e >> 4] = A[(m0 + (e >> 4)) * Kp + k0 + (e & 15)];
+            bs[e >> 6][e & 63] = B[(k0 + (e >> 6)) * Np + n0 + (e & 63)];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < kEstK; ++k) {
+            double a[4], b[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) a[i] = as[k][ty * 4 + i];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) b[j] = bs[k][tx * 4 + j];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_fma(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) C[(m0 + ty * 4 + i) * Np + n0 + tx * 4 + j] = acc[i][j];
+}
+
+// Kept columns of row i from the dense estimates (est_ij > thr, j >= i when
+// upper), ascending, at out_cols[out_off[i]..], and their count.
+__global__ __launch_bounds__(256) void k_est_compact(const double* __restrict__ est, int64_t ld, int nbc, double thr,
+                                                     int upper, const int64_t* __restrict__ out_off,
+                                                     int* __restrict__ out_cols, int* __restrict__ out_count) {
+    __shared__ int scan[256];
+    const int i = blockIdx.x, tid = threadIdx.x;
+    const double* row = est + int64_t(i) * ld;
+    const int C = (nbc + 255) / 256;
+    const int j0 = tid * C, j1 = j0 + C < nbc ? j0 + C : nbc;
+    auto keep = [&](int j) { return row[j] > thr && (!upper || j >= i); };
+    int mine = 0;
+    for (int j = j0; j < j1; ++j) mine += keep(j);
+    scan[tid] = mine;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        const int v = tid >= off ? scan[tid - off] : 0;
+        __syncthreads();
+        scan[tid] += v;
+        __syncthreads();
+    }
+    int* out = out_cols + out_off[i];
+    int pos = scan[tid] - mine;
+    for (int j = j0; j < j1; ++j)
+        if (keep(j)) out[pos++] = j;
+    if (tid == 255) out_count[i] = scan[255];
+}
+
 // Row i's kept columns, out_cols[out_off[i] .. + (crp[i+1] - crp[i])), to
 // cci[crp[i] ..]: one work-group per row.
 __global__ __launch_bounds__(256) void k_compact_cols(const int* __restrict__ out_cols,
@@ -1135,10 +1215,52 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
         sa.pairs = d_pairs;
         sa.thr = thr;
         sa.upper = upper;
+        // Dense symbolic phase when the candidate pattern is dense enough
+        // (the bound of every row capped at nbc covers >= 30 % of the
+        // matrix) and the three dense arrays fit half the free memory:
+        // estimates by one product of the block-norm matrices.
+        bool dense_sym = false;
+        {
+            double cover = 0.0;
+            for (int i = 0; i < nbr; ++i) cover += double(std::min<int64_t>(bound[i], B->nbc));
+            const int64_t Mp = (nbr + kEstTile - 1) / kEstTile * kEstTile;
+            const int64_t Np = (B->nbc + kEstTile - 1) / kEstTile * kEstTile;
+            const int64_t Kp = (A->nbc + kEstK - 1) / kEstK * kEstK;
+            if (thr >= 0.0 && nbr > 0 && B->nbc > 0 && cover >= 0.3 * double(nbr) * double(B->nbc) &&
+                (Mp / kEstTile) * (Np / kEstTile) <= INT32_MAX - 8) {
+                const size_t need = sizeof(double) * size_t(Mp * Kp + Kp * Np + Mp * Np);
+                size_t free_b = 0, total_b = 0;
+                VBH_CHECK_HIP(hipMemGetInfo(&free_b, &total_b));
+                dense_sym = need <= free_b / 2;
+            }
+            if (dense_sym) {
+                DevList dt;
+                double* na = dt.own((double*)dalloc(sizeof(double) * size_t(Mp * Kp)));
+                double* nbm = dt.own((double*)dalloc(sizeof(double) * size_t(Kp * Np)));
+                double* est = dt.own((double*)dalloc(sizeof(double) * size_t(Mp * Np)));
+                VBH_CHECK_HIP(hipMemsetAsync(na, 0, sizeof(double) * size_t(Mp * Kp), s));
+                VBH_CHECK_HIP(hipMemsetAsync(nbm, 0, sizeof(double) * size_t(Kp * Np), s));
+                hipLaunchKernelGGL(k_block_map<double>, dim3(nbr), dim3(256), 0, s, A->row_ptr, A->col_ind, nbr,
+                                   (const double*)an, Kp, na);
+                if (B->nbr > 0)
+                    hipLaunchKernelGGL(k_block_map<double>, dim3(B->nbr), dim3(256), 0, s, B->row_ptr, B->col_ind,
+                                       B->nbr, (const double*)bn, Np, nbm);
+                const int tiles_n = int(Np / kEstTile), n_tiles = int(Mp / kEstTile) * tiles_n;
+                hipLaunchKernelGGL(k_gemm_f64, dim3((n_tiles + 7) & ~7), dim3(256), 0, s, na, nbm, est, Kp, Np,
+                                   tiles_n, n_tiles);
+                hipLaunchKernelGGL(k_est_compact, dim3(nbr), dim3(256), 0, s, est, Np, B->nbc, thr, upper,
+                                   d_out_off, d_out_cols, d_count);
+                VBH_CHECK_HIP(hipGetLastError());
+                // candidate pairs per row: the bound (every B slot of every A slot)
+                VBH_CHECK_HIP(hipMemcpyAsync(d_pairs, bound.data(), sizeof(int64_t) * size_t(nbr),
+                                             hipMemcpyHostToDevice, s));
+                VBH_CHECK_HIP(hipStreamSynchronize(s));  // before the temporaries are freed
+            }
+        }
         // LDS classes: <=256, <=1024, <=4096 slots; wider rows use global tables
         const int classes[3] = {256, 1024, 4096};
         std::vector<int> rows_cls[4];
-        for (int i = 0; i < nbr; ++i) {
+        for (int i = 0; i < nbr && !dense_sym; ++i) {
             if (bound[i] == 0) continue;
             int c = 3;
             for (int q = 0; q < 3; ++q)
