@@ -112,6 +112,12 @@ VBH_API void* vbh_matrix_values(vbh_matrix_t m);
 VBH_API int vbh_matrix_values_changed(vbh_matrix_t m);
 /* Deep copy (block_csr.hpp:579 duplicate). */
 VBH_API int vbh_matrix_duplicate(vbh_matrix_t m, vbh_matrix_t* out, void* stream);
+/* Transpose (conj != 0: conjugate transpose) as a new matrix: nbc x nbr
+ * blocks, block (j, i) = A_ij^T (^H), rows sorted (BlockSpMat::transpose,
+ * block_csr.hpp:1530-1543 with detail/ops/transpose.hpp:178-).  The pattern
+ * comes from the handle's host mirror; the values are transposed on the
+ * device. */
+VBH_API int vbh_matrix_transpose(vbh_matrix_t m, int conj, vbh_matrix_t* out, void* stream);
 
 /* ---- applies ------------------------------------------------------------
  * precision: 0 = the matrix's own type (FP64 / complex128 vectors),

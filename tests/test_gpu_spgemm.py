@@ -215,3 +215,37 @@ def test_dense_panel_products(numeric, tol):
         assert got == P.spmm_stats[0]
         return
     assert abs(got - want) <= 1e-3 * want, (got, want)
+
+
+def test_device_transpose_and_trans_products(golden_complex):
+    """Conjugate transpose on the device (vbh_matrix_transpose) against the
+    dense conjugate transpose, for a non-symmetric variable-block matrix and
+    the reference's complex BSR golden; spmm with transA / transB against
+    the oracle on the explicit transposes (block_csr.hpp:1530-1543)."""
+    from vbcsr_amd import VBCSR
+    from vbcsr_amd import workloads as W
+
+    A = W.random_vbcsr(60, per_row=7, seed=3)  # not symmetric
+    H = VBCSR.from_block_csr(A.bs, A.row_ptr, A.col_ind, A.vals)
+    Ht = H.transpose()
+    assert np.array_equal(Ht.to_dense(), A.to_dense().T)
+    for i in range(A.nb):  # columns ascending per row
+        cols = Ht.col_ind[Ht.row_ptr[i]:Ht.row_ptr[i + 1]]
+        assert np.all(np.diff(cols) > 0)
+    d = golden_complex
+    G = from_golden(d, "G")
+    Gd = G.to_dense()
+    assert np.array_equal(G.transpose().to_dense(), Gd.conj().T)
+    # A^T A (transA) and A A^T (transB) against the oracle on the explicit
+    # transpose
+    At = O.Mat.of(W.BlockCSR(A.bs, Ht.row_ptr, Ht.col_ind, Ht.get_values()))
+    Am = O.Mat.of(A)
+    for thr in (0.0, 0.5):
+        C = H.spmm(H, thr, transA=True)
+        Cr = O.spgemm(At, Am, thr)
+        assert np.array_equal(C.row_ptr, Cr.row_ptr) and np.array_equal(C.col_ind, Cr.col_ind), thr
+        assert relerr(C.get_values(), Cr.vals) < 1e-12
+        C2 = H.spmm(H, thr, transB=True)
+        Cr2 = O.spgemm(Am, At, thr)
+        assert np.array_equal(C2.row_ptr, Cr2.row_ptr) and np.array_equal(C2.col_ind, Cr2.col_ind), thr
+        assert relerr(C2.get_values(), Cr2.vals) < 1e-12

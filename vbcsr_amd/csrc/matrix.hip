@@ -621,6 +621,28 @@ vbh_matrix* filter_by_norms(vbh_matrix* m, double thr, const double* d_norm, boo
     return g.release();
 }
 
+// Block q of the transpose is source slot src[q] (r x c) transposed (and
+// conjugated): out[b * r + a] = conj(in[a * c + b]); one wavefront per block.
+template <typename T>
+__global__ __launch_bounds__(256) void k_transpose_blocks(const T* __restrict__ in, const int64_t* __restrict__ in_voff,
+                                                          const int* __restrict__ src, const int* __restrict__ src_cols,
+                                                          const int* __restrict__ in_cbs, T* __restrict__ out,
+                                                          const int64_t* __restrict__ out_voff, int64_t nq, int conj) {
+    const int64_t q = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (q >= nq) return;
+    const int k = src[q];
+    const int c = in_cbs[src_cols[q]];
+    const int64_t ib = in_voff[k], ob = out_voff[q];
+    const int64_t n = out_voff[q + 1] - ob;
+    const int r = int(n / c);
+    for (int64_t e = lane; e < n; e += 64) {
+        const int b = int(e / r), a = int(e - int64_t(b) * r);  // out element (b, a)
+        const T v = in[ib + int64_t(a) * c + b];
+        out[ob + e] = conj ? Sc<T>::conj(v) : v;
+    }
+}
+
 }  // namespace vbh
 
 using namespace vbh;
@@ -799,6 +821,44 @@ static vbh_matrix* clone_matrix(const vbh_matrix* m, hipStream_t s) {
         throw;
     }
     return c;
+}
+
+VBH_API int vbh_matrix_transpose(vbh_matrix_t m, int conj, vbh_matrix_t* out, void* stream) {
+    return guard_on(stream, [&] {
+        VBH_REQUIRE(m && out, "null argument");
+        hipStream_t s = (hipStream_t)stream;
+        // pattern: the column-major order of the host mirror (rows ascending
+        // within each column), source slots and their block columns
+        const int nbr = m->nbr, nbc = m->nbc;
+        std::vector<int> tp(nbc + 1, 0), tr(m->nnzb), src(m->nnzb), scol(m->nnzb);
+        for (int64_t k = 0; k < m->nnzb; ++k) tp[m->h_col_ind[k] + 1]++;
+        for (int j = 0; j < nbc; ++j) tp[j + 1] += tp[j];
+        std::vector<int> fill(tp.begin(), tp.end() - 1);
+        for (int i = 0; i < nbr; ++i)
+            for (int k = m->h_row_ptr[i]; k < m->h_row_ptr[i + 1]; ++k) {
+                const int j = m->h_col_ind[k];
+                const int q = fill[j]++;
+                tr[q] = i;
+                src[q] = k;
+                scol[q] = j;
+            }
+        MatrixGuard g(build_matrix(nbc, nbr, m->h_cbs.data(), m->h_rbs.data(), tp.data(), tr.data(), m->dtype, s));
+        vbh_matrix* t = g.m;
+        if (t->nnzb) {
+            DevList tmp;
+            int* d_src = tmp.own(upload(src, s));
+            int* d_scol = tmp.own(upload(scol, s));
+            dispatch_dtype(m->dtype, [&](auto tag) {
+                using T = decltype(tag);
+                hipLaunchKernelGGL(k_transpose_blocks<T>, dim3(blocks_for(t->nnzb, 4)), dim3(256), 0, s,
+                                   (const T*)m->vals, m->voff, d_src, d_scol, m->cbs, (T*)t->vals, t->voff, t->nnzb,
+                                   conj);
+            });
+            VBH_CHECK_HIP(hipGetLastError());
+            VBH_CHECK_HIP(hipStreamSynchronize(s));
+        }
+        *out = g.release();
+    });
 }
 
 VBH_API int vbh_matrix_duplicate(vbh_matrix_t m, vbh_matrix_t* out, void* stream) {

@@ -728,35 +728,18 @@ struct Mat {
         g->finish();
         return g;
     }
-    // Conjugate transpose (host pattern transpose + value gather): square
-    // local structure without ghosts.
+    // Conjugate transpose on the device (vbh_matrix_transpose): square local
+    // structure without ghosts.
     std::shared_ptr<Mat> transpose() {
         if (graph->ghost_global_indices.size())
             throw std::runtime_error("transpose: ghost columns need the distributed exchange");
-        to_host();
-        const int n = graph->n_owned();
-        std::vector<std::vector<int>> adj(n);
-        for (int i = 0; i < n; ++i)
-            for (int k = graph->adj_ptr[i]; k < graph->adj_ptr[i + 1]; ++k)
-                adj[graph->adj_ind[k]].push_back(graph->owned_global_indices[i]);
-        auto g = std::make_shared<Graph>();
-        std::vector<int> sizes(graph->block_sizes.begin(), graph->block_sizes.begin() + n);
-        g->build(graph->owned_global_indices, sizes, adj, {}, {});
-        auto t = std::make_shared<Mat>(g);
-        for (int i = 0; i < n; ++i)
-            for (int k = graph->adj_ptr[i]; k < graph->adj_ptr[i + 1]; ++k) {
-                const int j = graph->adj_ind[k];
-                const int r = graph->block_sizes[i], c = graph->block_sizes[j];
-                const int ts = t->find_slot(j, i);
-                for (int a = 0; a < r; ++a)
-                    for (int b = 0; b < c; ++b) {
-                        T v = host[voff[k] + int64_t(a) * c + b];
-                        if constexpr (std::is_same_v<T, cplx>) v = std::conj(v);
-                        t->host[t->voff[ts] + int64_t(b) * r + a] = v;
-                    }
-            }
-        t->host_written();
-        return t;
+        vbh_matrix_t out = nullptr;
+        vbh_matrix_t a = dev();
+        {
+            py::gil_scoped_release nogil;
+            check(vbh_matrix_transpose(a, std::is_same_v<T, cplx> ? 1 : 0, &out, nullptr), "transpose");
+        }
+        return std::make_shared<Mat>(graph_for_pattern(out, *graph), out);
     }
     py::array_t<T> get_values() {
         to_host();
