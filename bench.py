@@ -51,7 +51,9 @@ def parse():
     p.add_argument("--only", default="", help="comma list: kpm,spmv,batched,prop,mcweeny,cpu (default all)")
     p.add_argument("--mcweeny-iters", type=int, default=5,
                    help="McWeeny iterations per precision (the configured 10 fill P to ~0.9 G values and take "
-                        "minutes per precision; 5 keep the default run short)")
+                        "9-40 s per precision; 5 keep the default run short)")
+    p.add_argument("--mcweeny-legs", default="fp64,fp64_hermitian,fp32,bf16",
+                   help="comma list of McWeeny legs (the first one is the reference of the others' errors)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a 1 GiB device-to-device copy 5 times (scale for the HBM byte counters)")
     return p.parse_args()
@@ -305,7 +307,11 @@ def bench_mcweeny(args, sync, log):
     P_ref = None
     # "fp64_hermitian": both products by spmm_hermitian (P, P^2 and P^3 are
     # symmetric: only the upper block triangle is multiplied, then mirrored)
-    for leg in ("fp64", "fp64_hermitian", "fp32", "bf16"):
+    legs = [g for g in args.mcweeny_legs.split(",") if g]
+    bad = set(legs) - {"fp64", "fp64_hermitian", "fp32", "bf16"}
+    if bad:
+        raise SystemExit(f"unknown McWeeny legs {sorted(bad)}")
+    for leg in legs:
         numeric = leg.split("_")[0]
         herm = leg.endswith("hermitian")
         P = VBCSR.from_block_csr(P0.bs, P0.row_ptr, P0.col_ind, P0.vals)
@@ -348,11 +354,11 @@ def bench_mcweeny(args, sync, log):
             D = (P - P_ref).get_values()
             res[leg]["vs_fp64"] = {"max_abs": float(np.abs(D).max()) if D.size else 0.0,
                                    "frobenius": float(np.linalg.norm(D)),
-                                   "trace_diff": res[leg]["trace"] - res["fp64"]["trace"]}
+                                   "trace_diff": res[leg]["trace"] - res[legs[0]]["trace"], "against": legs[0]}
         log(f"mcweeny {leg}: {done} iterations in {secs:.2f} s, trace {res[leg]['trace']:.6f}, "
             f"||P^2-P||_F {idem:.3e}" + (f", vs fp64 max {res[leg]['vs_fp64']['max_abs']:.2e}"
                                           if "vs_fp64" in res[leg] else ""))
-    if not args.no_cpu and not args.quick:
+    if not args.no_cpu and not args.quick and "fp64" in res:
         cpu = mcweeny_reference_cpu(P0, thr, iters, 12.0, log)
         if cpu:
             n = len(cpu)

@@ -191,13 +191,25 @@ def test_dense_panel_products(numeric, tol):
         p3 = O.spgemm(p2, pm, thr)
         assert np.array_equal(P3.row_ptr, p3.row_ptr) and np.array_equal(P3.col_ind, p3.col_ind), thr
         assert relerr(P3.get_values(), p3.vals) < tol, thr
-    # numeric pairs: (i, k, j) with A_ik and B_kj stored, C_ij in the symbolic
-    # pattern (est_ij > thr) and ||A_ik|| ||B_kj|| >= thr / nnz_row(A_i)
-    # (ops/spmm/vbcsr.hpp:371-387), counted on the CPU from the block norms
-    nb, rp, thr = P0.nb, P0.row_ptr, 1e-5
+    thr = 1e-5
+    want = _numeric_pairs(P0, thr)
+    P.spmm(P, thr, numeric=numeric)
+    got = P.spmm_stats[1]
+    if numeric == "bf16":  # the dense BF16 products multiply every candidate pair
+        assert got == P.spmm_stats[0]
+        return
+    assert abs(got - want) <= 1e-3 * want, (got, want)
+
+
+def _numeric_pairs(P0, thr):
+    """Numeric pairs of P0 P0: (i, k, j) with A_ik and B_kj stored, C_ij in
+    the symbolic pattern (est_ij > thr) and ||A_ik|| ||B_kj|| >= thr /
+    nnz_row(A_i) (ops/spmm/vbcsr.hpp:371-387), counted on the CPU from the
+    block norms."""
+    nb, rp = P0.nb, P0.row_ptr
     nrm = np.zeros((nb, nb))
     pres = np.zeros((nb, nb), bool)
-    bn = O.block_norms(pm)
+    bn = O.block_norms(O.Mat.of(P0))
     for i in range(nb):
         for k in range(rp[i], rp[i + 1]):
             nrm[i, P0.col_ind[k]] = bn[k]
@@ -209,12 +221,35 @@ def test_dense_panel_products(numeric, tol):
         prod = nrm[i][:, None] * nrm  # [k, j]
         ok = pres[i][:, None] & pres & sym[i][None, :] & ~(prod < eps[i])
         want += int(ok.sum())
-    P.spmm(P, thr, numeric=numeric)
-    got = P.spmm_stats[1]
-    if numeric == "bf16":  # the dense BF16 products multiply every candidate pair
-        assert got == P.spmm_stats[0]
-        return
-    assert abs(got - want) <= 1e-3 * want, (got, want)
+    return want
+
+
+@pytest.mark.parametrize("numeric,tol", [("fp64", 1e-12), ("fp32", 1e-5), ("bf16", 2e-2)])
+def test_sparse_products_fused_filter_and_pairs(numeric, tol):
+    """Sparse products (C far below the dense paths' fill): the numeric
+    kernels write C's block norms for the threshold filter themselves (and
+    the FP32 kernel widens its FP32 accumulators into C in the same pass).
+    The filtered pattern and values against the oracle, and the numeric pair
+    count of the reference's rule, for all three numeric types."""
+    from vbcsr_amd import VBCSR
+    from vbcsr_amd import workloads as W
+
+    P0 = W.mcweeny_start(nb=600, neighbours=8.0, seed=9, normalize="spectral")  # C fill ~0.05
+    P = VBCSR.from_block_csr(P0.bs, P0.row_ptr, P0.col_ind, P0.vals)
+    pm = O.Mat.of(P0)
+    for thr in (1e-3, 1e-5):
+        C = P.spmm(P, thr, numeric=numeric)
+        st = P.spmm_stats
+        assert st[2] < 0.1 * P0.nb * P0.nb, st  # the sparse kernels, not the dense paths
+        assert st[3] <= st[2], st
+        ref = O.spgemm(pm, pm, thr)
+        if numeric == "bf16":  # blocks near the threshold may flip under BF16 rounding
+            assert relerr(C.to_dense(), W.BlockCSR(P0.bs, ref.row_ptr, ref.col_ind, ref.vals).to_dense()) < tol
+        else:
+            assert np.array_equal(C.row_ptr, ref.row_ptr) and np.array_equal(C.col_ind, ref.col_ind), thr
+            assert relerr(C.get_values(), ref.vals) < tol, thr
+        want = _numeric_pairs(P0, thr)
+        assert abs(int(st[1]) - want) <= 1e-3 * want, (numeric, thr, int(st[1]), want)
 
 
 def test_device_transpose_and_trans_products(golden_complex):

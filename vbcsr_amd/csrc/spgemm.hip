@@ -302,6 +302,7 @@ struct NumArgs {
     const CT* a_c;
     const CT* b_c;
     CT* c_c;
+    double* c_norm;  // [nnzb(C)] block Frobenius norms for the filter, or null
 };
 
 __device__ inline int find_col(const int* __restrict__ cols, int lo, int hi, int j) {
@@ -329,6 +330,35 @@ __device__ inline T widen(CT x) {
     if constexpr (std::is_same_v<T, CT>) return x;
     else if constexpr (Sc<T>::is_complex) return T{double(x.re), double(x.im)};
     else return T(x);
+}
+
+// Frobenius norms of C row i's blocks from the values a numeric kernel has
+// just written (the filter's norm pass, fused into the numeric phase): wave
+// w of nw takes the row's slots c0 + w, c0 + w + nw, ...; lane l sums the
+// squares of elements l, l + 64, ... of a block, then a butterfly over the
+// wave -- the order of k_block_norms, so the filter keeps exactly the blocks
+// a separate norm pass would keep.  With NS the block is read from the
+// narrow accumulation array and widened into C's values on the way.
+template <typename T, typename CT, bool NS>
+__device__ inline void row_block_norms(const CT* __restrict__ src, T* __restrict__ vals,
+                                       const int64_t* __restrict__ voff, int c0, int c1, int wave, int nw, int lane,
+                                       double* __restrict__ norm) {
+    for (int q = c0 + wave; q < c1; q += nw) {
+        const int64_t b = voff[q], e = voff[q + 1];
+        double sum = 0.0;
+        for (int64_t x = b + lane; x < e; x += 64) {
+            T v;
+            if constexpr (NS) {
+                v = widen<T, CT>(src[x]);
+                vals[x] = v;
+            } else {
+                v = vals[x];
+            }
+            sum += sqabs(v);
+        }
+        for (int m = 32; m >= 1; m >>= 1) sum += __shfl_xor(sum, m, 64);
+        if (lane == 0 && norm) norm[q] = sqrt(sum);
+    }
 }
 
 template <typename T, typename CT>
@@ -367,6 +397,7 @@ __global__ __launch_bounds__(256) void k_numeric(NumArgs<T, CT> a) {
         __syncthreads();
     }
     if (lane == 0 && used) atomicAdd(&a.npairs[0], used);
+    if (a.c_norm) row_block_norms<T, CT, false>(nullptr, a.c_vals, a.c_voff, c0, c1, wave, 4, lane, a.c_norm);
 }
 
 // Numeric phase for blocks up to 8 x 8 (the common case): C slot q of the row
@@ -522,13 +553,11 @@ __global__ __launch_bounds__(64 * kNumWaves) void k_numeric_owned(NumArgs<T, CT>
         }
     }
     if (lane == 0 && used) atomicAdd(&a.npairs[0], used);
-}
-
-// c_vals = widen(c_c) after an FP32-product numeric phase.
-template <typename T, typename CT>
-__global__ void k_widen_values(const CT* __restrict__ in, T* __restrict__ out, int64_t n) {
-    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = widen<T, CT>(in[i]);
+    // the wave's own slots: widened into C (narrow storage) and their norms
+    if (NS || a.c_norm) {
+        __syncthreads();
+        row_block_norms<T, CT, NS>(a.c_c, a.c_vals, a.c_voff, c0, c1, wave, kNumWaves, lane, a.c_norm);
+    }
 }
 
 // ------------------------------------------------------------ BF16 MFMA numeric
@@ -571,6 +600,8 @@ struct MfmaArgs {
     const int* panel_width; // per C row: total panel columns
     double* c_vals;
     double thr;
+    double* c_norm;                // [nnzb(C)] block norms for the filter, or null
+    unsigned long long* npairs;    // multiplied (A slot, C block) pairs
 };
 
 // Panel column -> (C slot, column within block), found by binary search over
@@ -601,6 +632,7 @@ __global__ __launch_bounds__(256) void k_numeric_mfma_bf16(MfmaArgs a) {
     // (l % 16), rows 4*(l / 16) .. +3.
     const int lr = lane & 15;
     const int lk = (lane >> 4) * 8;
+    unsigned long long used = 0;
     for (int pc0 = wave * 16; pc0 < width; pc0 += 64) {
         const int pc = pc0 + lr;  // this lane's panel column (B operand / result)
         int slot = -1, jcol = -1, cb = 0, cwidth = 0;
@@ -624,6 +656,8 @@ __global__ __launch_bounds__(256) void k_numeric_mfma_bf16(MfmaArgs a) {
                 if (pb < b1 && a.b_col_ind[pb] == jcol && !(na * a.b_norm[pb] < row_eps))
                     Bcol = a.b_vals + a.b_voff[pb] + cb;
             }
+            // one lane per (A slot, C block): the block's first column in lane group 0
+            used += __popcll(__ballot(Bcol != nullptr && cb == 0 && lane < 16));
             for (int kk = 0; kk < m; kk += 32) {
                 bf16x8 av, bv;
 #pragma unroll
@@ -643,6 +677,11 @@ __global__ __launch_bounds__(256) void k_numeric_mfma_bf16(MfmaArgs a) {
                 if (row < r) C[row * cwidth + cb] = double(acc[t]);
             }
         }
+    }
+    if (lane == 0 && used) atomicAdd(a.npairs, used);
+    if (a.c_norm) {
+        __syncthreads();  // every wave's columns of the row are written
+        row_block_norms<double, double, false>(nullptr, a.c_vals, a.c_voff, c0, c1, wave, 4, lane, a.c_norm);
     }
 }
 
@@ -672,6 +711,7 @@ __global__ __launch_bounds__(256) void k_numeric_mfma_bf16_map(MfmaArgs a, int n
     const int lk = (lane >> 4) * 8;
     constexpr int S = kMfmaStrips;
     const int passes = (width + 64 * S - 1) / (64 * S);  // the same for every wave (barriers below)
+    unsigned long long used = 0;
     for (int p = 0; p < passes; ++p) {
         int slot[S], jcol[S], cb[S], cwidth[S];
 #pragma unroll
@@ -714,6 +754,8 @@ __global__ __launch_bounds__(256) void k_numeric_mfma_bf16_map(MfmaArgs a, int n
                         const int2 e = bmap[jcol[t]];
                         if (e.x == ka) Bcol = a.b_vals + a.b_voff[e.y] + cb[t];
                     }
+                    // one lane per (A slot, C block): the block's first column in lane group 0
+                    if (kk == 0) used += __popcll(__ballot(Bcol != nullptr && cb[t] == 0 && lane < 16));
                     // a strip with no B block of this A slot would add exact
                     // zeros: the wave skips its loads and product
                     if (__ballot(Bcol != nullptr) == 0) continue;
@@ -738,6 +780,11 @@ __global__ __launch_bounds__(256) void k_numeric_mfma_bf16_map(MfmaArgs a, int n
                 }
             }
         }
+    }
+    if (lane == 0 && used) atomicAdd(a.npairs, used);
+    if (a.c_norm) {
+        __syncthreads();  // every wave's strips of the row are written
+        row_block_norms<double, double, false>(nullptr, a.c_vals, a.c_voff, c0, c1, wave, 4, lane, a.c_norm);
     }
 }
 
@@ -1343,6 +1390,7 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
         VBH_CHECK_HIP(hipMemsetAsync(d_np, 0, sizeof(unsigned long long), s));
         // ---- numeric
         double* d_cnorm2 = nullptr;  // squared C block norms from the panel kernel (fused filter)
+        double* d_cnorm = nullptr;   // C block norms from the sparse kernels (fused filter)
         const double fill = double(Cm->nnzb) / (double(std::max(nbr, 1)) * double(std::max(Cm->nbc, 1)));
         bool panel = numeric != 2 && Cm->nnzb > 0 && A->dtype == VBH_F64 && A->max_block <= 8 &&
                      B->max_block <= 8 && fill >= kPanelFill;
@@ -1480,6 +1528,8 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
             if (numeric == 1) go((float*)nullptr);
             else go((double*)nullptr);
         } else if (Cm->nnzb > 0) {
+            // the sparse kernels write C's block norms for the filter (fused)
+            if (thr > 0.0) d_cnorm = tmp.own((double*)dalloc(sizeof(double) * size_t(Cm->nnzb)));
             if (numeric == 2) {
                 std::vector<int> colstart(Cm->nnzb), width(nbr);
                 for (int i = 0; i < nbr; ++i) {
@@ -1494,7 +1544,7 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
                 int* d_w = tmp.own(upload(width, s));
                 MfmaArgs ma{A->row_ptr, A->col_ind, A->voff, (const double*)A->vals, an, A->rbs, A->cbs,
                             B->row_ptr, B->col_ind, B->voff, (const double*)B->vals, bn, Cm->row_ptr, Cm->col_ind,
-                            Cm->voff, Cm->cbs, d_cs, d_w, (double*)Cm->vals, thr};
+                            Cm->voff, Cm->cbs, d_cs, d_w, (double*)Cm->vals, thr, d_cnorm, d_np};
                 if (Cm->nbc <= kNumMapCols)
                     hipLaunchKernelGGL(k_numeric_mfma_bf16_map, dim3(nbr), dim3(256), sizeof(int2) * size_t(Cm->nbc), s,
                                        ma, Cm->nbc);
@@ -1509,7 +1559,7 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
                     NumArgs<T, CT> na{A->row_ptr, A->col_ind, A->voff, (const T*)A->vals, an, A->rbs, A->cbs,
                                       B->row_ptr, B->col_ind, B->voff, (const T*)B->vals, bn, Cm->row_ptr,
                                       Cm->col_ind, Cm->voff, Cm->cbs, (T*)Cm->vals, d_np, thr,
-                                      nullptr, nullptr, nullptr};
+                                      nullptr, nullptr, nullptr, d_cnorm};
                     if (A->max_block <= 8 && B->max_block <= 8) {
                         constexpr bool narrow_products = !std::is_same_v<T, CT>;
                         if constexpr (narrow_products) {
@@ -1530,10 +1580,7 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
                                                Cm->nbc);
                         if constexpr (narrow_products) {
                             VBH_CHECK_HIP(hipGetLastError());
-                            hipLaunchKernelGGL((k_widen_values<T, CT>), dim3(unsigned((Cm->nnz + 255) / 256)), dim3(256),
-                                               0, s, na.c_c, (T*)Cm->vals, Cm->nnz);
-                            VBH_CHECK_HIP(hipGetLastError());
-                            VBH_CHECK_HIP(hipStreamSynchronize(s));
+                            VBH_CHECK_HIP(hipStreamSynchronize(s));  // before c_c is freed
                         }
                     } else {
                         hipLaunchKernelGGL((k_numeric<T, CT>), dim3(nbr), dim3(256), 0, s, na);
@@ -1556,9 +1603,9 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
         // ---- filter
         vbh_matrix* F = nullptr;
         if (thr > 0.0 && C.m->nnzb > 0) {
-            if (d_cnorm2) {
-                // norms from the numeric phase's registers: no separate pass
-                F = filter_by_norms(C.m, thr, d_cnorm2, true, s);
+            if (d_cnorm2 || d_cnorm) {
+                // norms from the numeric phase: no separate pass
+                F = d_cnorm2 ? filter_by_norms(C.m, thr, d_cnorm2, true, s) : filter_by_norms(C.m, thr, d_cnorm, false, s);
             } else {
                 const int rc = vbh_matrix_filter(C.m, thr, &F, stream);
                 if (rc != VBH_OK) throw Error(rc, vbh_last_error());
