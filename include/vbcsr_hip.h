@@ -191,23 +191,29 @@ VBH_API int vbh_kpm_moments_batched(int count, const vbh_matrix_t* ms, int K, co
 VBH_API int vbh_kpm_last_step_time(double* ms, int64_t* launches);
 /* Tile autotuning of vbh_kpm_moments_batched (on by default).  When on, the
  * first call on a batch (same matrix handles, K, precision, mode) times one
- * recursion step for each candidate plan -- the matrix-core threshold (block
- * rows of t..16 rows on the FP64 matrix cores, t in {4, 8, 17 = none}) and the
- * kernel of the other rows: the lane family with its slot partitions, or the
- * row-wave kernel (one scalar row per wavefront, real types) with its rows per
- * work-group -- and keeps the fastest for every later call on that batch.  Plans change the order of floating-point sums only.  No
- * reference counterpart (the reference has no device tiles). */
+ * recursion step for each candidate plan -- the class threshold (block rows
+ * of t..16 rows in the class, t in {4, 8, 17 = none}), the kernel of the
+ * class rows (the FP64 matrix cores, or for FP32 values and K in 16, 32, 64
+ * the block-row wave kernel on the vector ALUs), and the kernel of the other
+ * rows: the lane family with its slot partitions, or the row-wave kernel (one
+ * scalar row per wavefront, real types) with its rows per work-group -- and
+ * keeps the fastest for every later call on that batch.  Plans change the
+ * order of floating-point sums only.  No reference counterpart (the reference
+ * has no device tiles). */
 VBH_API int vbh_set_autotune(int on);
-/* Pin the plan of later vbh_kpm_moments_batched calls (threshold 1..17;
- * parts: a power of two in 1..64 = lane-family slot partitions, clamped to
- * the lane shape, or its negative = row-wave kernel with -parts rows per
- * work-group, where that kernel serves the call), bypassing the cache and the
- * autotuner; threshold 0 releases it.  For reproducible
- * runs and for tests of every plan.  No reference counterpart. */
+/* Pin the plan of later vbh_kpm_moments_batched calls (threshold 1..17: the
+ * class rows on the matrix cores, -16..-1: class rows of -threshold..16 rows
+ * on the block-row wave kernel, where that kernel serves the call; parts: a
+ * power of two in 1..64 = lane-family slot partitions, clamped to the lane
+ * shape, or its negative = row-wave kernel with -parts rows per work-group,
+ * where that kernel serves the call), bypassing the cache and the autotuner;
+ * threshold 0 releases it.  For reproducible runs and for tests of every
+ * plan.  No reference counterpart. */
 VBH_API int vbh_set_batch_plan(int threshold, int parts);
-/* Plan of the last vbh_kpm_moments_batched call: matrix-core threshold, slot
- * partitions (negative: row-wave rows per work-group), and whether it came
- * from timing (1) or the default rule (0). */
+/* Plan of the last vbh_kpm_moments_batched call: class threshold (negative:
+ * class rows on the block-row wave kernel), slot partitions (negative:
+ * row-wave rows per work-group), and whether it came from timing (1) or the
+ * default rule (0). */
 VBH_API int vbh_kpm_last_plan(int* threshold, int* parts, int* tuned);
 /* Chebyshev series Y = sum_n coef_n T_n(H~) X (complex coefficients as
  * interleaved doubles, nterms of them; real matrices use the real parts).

@@ -1,6 +1,8 @@
 """GPU checks of the batched-KPM tile plans: every plan the autotuner can
-pick (matrix-core threshold x lane-family slot partitions or row-wave rows
-per work-group, the latter given as negative parts) gives the oracle's moments
+pick (class threshold, the class rows on the matrix cores or -- negative
+thresholds -- on the block-row wave kernel, x lane-family slot partitions or
+row-wave rows per work-group, the latter given as negative parts) gives the
+oracle's moments
 (FP64: 1e-11 relative; precision "mixed": 1e-11 against the oracle on the
 FP32-rounded matrix), the autotuner reports a timed plan from its candidate
 set, and a second call on the same batch reuses that plan bit for bit."""
@@ -60,7 +62,7 @@ def test_autotuner_picks_a_candidate_and_keeps_it(kpm_api):
     M = 24
     mu1 = K.kpm_moments_batched(mats, V0s, M, mode="two", precision="mixed")
     t, p, tuned = K.last_plan()
-    assert tuned and t in (4, 8, 17) and (p >= 1 or p in (-8, -16, -32))
+    assert tuned and t in (4, 8, 17, -4, -8) and (p >= 1 or p in (-8, -16, -32))
     mu2 = K.kpm_moments_batched(mats, V0s, M, mode="two", precision="mixed")
     assert K.last_plan() == (t, p, True)
     assert np.array_equal(mu1, mu2)
@@ -150,3 +152,44 @@ def test_row_wave_plans_wide(kpm_api, K, threshold):
             A = Hs[h] if precision == "fp64" else rounded32(Hs[h])
             ref = O.kpm_moments(O.Mat.of(A), np.asarray(V0s[h].to_numpy()), M, mode=1)
             assert relerr(mu[h], ref) < tol, (precision, h)
+
+
+@pytest.mark.parametrize("K", [16, 32, 64])
+@pytest.mark.parametrize("threshold", [-4, -8, -1])
+def test_block_row_wave_plans(kpm_api, K, threshold):
+    """The class rows on the block-row wave kernel (vector ALUs, DPP row
+    broadcasts of the column-major FP32 values) at the column counts it
+    serves, with the other rows on the lane family or the row-wave kernel;
+    threshold -1 puts every block row in the class (no lane launch).  Long
+    rows (about 70 neighbours) and rows of every size 1..13.  Precision
+    "mixed" against the oracle on the FP32-rounded matrix (1e-11), FP32
+    vectors at 2e-5; graph and plain launches."""
+    from vbcsr_amd import workloads as W
+
+    Kapi = kpm_api
+    Hs = W.batched_hamiltonians(count=3, nb_lo=60, nb_hi=200, neighbours=70.0, seed=43,
+                                sizes=(1, 2, 3, 4, 5, 7, 9, 13))
+    mats = [mk(A) for A in Hs]
+    V0s = []
+    for h, H in enumerate(mats):
+        v = H.create_multivector(K)
+        v.set_random(700 + h, normalize=False)
+        V0s.append(v)
+    M = 12
+    for parts in (1, -16):
+        Kapi.set_batch_plan(threshold, parts)
+        for precision, tol, graph in (("mixed", 1e-11, True), ("mixed", 1e-11, False), ("fp32", 2e-5, True)):
+            mu = Kapi.kpm_moments_batched(mats, V0s, M, mode="two", precision=precision, graph=graph)
+            assert Kapi.last_plan()[:2] == (threshold, parts), (precision, Kapi.last_plan())
+            for h in range(len(mats)):
+                ref = O.kpm_moments(O.Mat.of(rounded32(Hs[h])), np.asarray(V0s[h].to_numpy()), M, mode=1)
+                assert relerr(mu[h], ref) < tol, (parts, precision, graph, h)
+    # values changed after the column-major copy was built: it follows them
+    Kapi.set_batch_plan(threshold, 1)
+    for H in mats:
+        H.scale(0.5)
+    mu = Kapi.kpm_moments_batched(mats, V0s, M, mode="two", precision="mixed")
+    for h in range(len(mats)):
+        A = W.BlockCSR(Hs[h].bs, Hs[h].row_ptr, Hs[h].col_ind, 0.5 * Hs[h].vals)
+        ref = O.kpm_moments(O.Mat.of(rounded32(A)), np.asarray(V0s[h].to_numpy()), M, mode=1)
+        assert relerr(mu[h], ref) < 1e-11, h

@@ -433,6 +433,7 @@ struct LaneShape {
     int S, KT;   // S == 0: family not applicable
     int P = 1;   // partitions of each row's slots (lane_parts)
     int rw = 0;  // > 0: the row-wave kernel instead, rw scalar rows per work-group
+    int brw = 0;  // 1: the matrix-core class rows on the block-row wave kernel (vector ALUs)
 };
 // Columns per lane of the row-wave kernel (16 bytes of a row), or 0 when it
 // does not serve (T, K): real types, K a multiple of it with K / it a power
@@ -446,6 +447,16 @@ inline int rowwave_cpl(int K) {
         if (K < cpl || K % cpl) return 0;
         const int lq = K / cpl;
         return (lq & (lq - 1)) == 0 && lq <= 64 ? cpl : 0;
+    }
+}
+// Columns per lane of the block-row wave kernel (K = 16 * it: 1, 2 or 4), or
+// 0 when it does not serve (T, K): real types only.
+template <typename T>
+inline int brw_cpl(int K) {
+    if constexpr (Sc<T>::is_complex) {
+        return 0;
+    } else {
+        return (K == 16 || K == 32 || K == 64) ? K / 16 : 0;
     }
 }
 // Instantiated shapes: real K <= 16 one lane per row (KT in 4, 8, 10, 12, 16),
@@ -571,6 +582,12 @@ struct BatchItem {
     int G;              // this matrix's lane (or row-loop) work-groups
     int g_base;         // its first work-group in the lane launch
     int g_base2;        // its first work-group in the matrix-core launch
+    // block-row wave kernel (class rows on the vector ALUs): the column-major
+    // FP32 values and the X row of every stored scalar column of each block
+    // row (entries eoff[i] .. eoff[i+1])
+    const float* vt;
+    const int64_t* eoff;
+    const int* ecol;
 };
 template <typename T, typename V>
 void launch_batched_lanes(const BatchItem<T, V>* items, int nitems, int Gtot, int K, int ld, LaneShape sh, int step,
@@ -580,6 +597,9 @@ template <typename T, typename V>
 void launch_batched_rowwave(const BatchItem<T, V>* items, int nitems, int Gtot, int K, int ld, int rw, int step,
                             double alpha, double beta, double gamma0, double gamma, int mode, double* partials,
                             hipStream_t s);
+template <typename T, typename V>
+void launch_batched_brw(const BatchItem<T, V>* items, int nitems, int Gtot, int K, int ld, int step, double alpha,
+                        double beta, double gamma0, double gamma, int mode, double* partials, hipStream_t s);
 template <typename T, typename V>
 void launch_batched_mfma(const BatchItem<T, V>* items, int nitems, int Gtot, int K, int ld, int step, double alpha,
                          double beta, double gamma0, double gamma, int mode, double* partials, hipStream_t s);

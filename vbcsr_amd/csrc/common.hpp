@@ -251,6 +251,14 @@ struct vbh_matrix {
     void* vals = nullptr;
     void* vals32 = nullptr;  // FP32 copy, built on demand
     bool vals32_valid = false;
+    // block-row wave kernel, built on demand: FP32 values with every block
+    // column-major (same offsets voff), valid while the values are unchanged,
+    // and the X row of every stored scalar column of each block row
+    // (entries eoff[i] .. eoff[i+1]; pattern only)
+    float* vt = nullptr;
+    bool vt_valid = false;
+    int64_t* eoff = nullptr;
+    int* ecol = nullptr;
     // apply plan: work-balanced chunks of block rows, G chunks (G % 8 == 0)
     int* split = nullptr;
     int G = 0;
@@ -284,8 +292,11 @@ T* upload(const std::vector<T>& h, hipStream_t s = nullptr) {
 }
 // Ensure the FP32 copy of the values exists and is current.
 void ensure_vals32(vbh_matrix* m, hipStream_t s);
-// Mark the device values as changed: the FP32 copy is rebuilt on next use.
+// Mark the device values as changed: the FP32 copies are rebuilt on next use.
 void values_changed(vbh_matrix* m);
+// Ensure the block-row wave kernel's column-major FP32 values and entry
+// columns exist and are current (real matrices).
+void ensure_brw(vbh_matrix* m, hipStream_t s);
 // Whether the FP32 copy is current / the row-class lists of threshold t exist
 // (so using them allocates nothing).
 bool has_vals32(vbh_matrix* m);

@@ -428,10 +428,11 @@ static BatchLayout batch_layout(int count, vbh_matrix* const* ms, const vbh_matr
     return L;
 }
 
-// Tile plan of a batch: the matrix-core threshold (block rows of t..16 rows
-// on the matrix cores, t = 17: none) and the kernel of the other rows: the
-// lane family with P slot partitions, or the row-wave kernel with rw rows per
-// work-group.  With autotuning on, the first call on a batch times one
+// Tile plan of a batch: the class threshold (block rows of t..16 rows in the
+// class, t = 17: none), the kernel of the class rows (the FP64 matrix cores,
+// or with brw the block-row wave kernel on the vector ALUs) and the kernel of
+// the other rows: the lane family with P slot partitions, or the row-wave
+// kernel with rw rows per work-group.  With autotuning on, the first call on a batch times one
 // recursion step of every candidate and keeps the fastest for later calls on
 // the same batch.
 struct BatchPlan {
@@ -439,13 +440,14 @@ struct BatchPlan {
     int P = 1;
     bool tuned = false;
     int rw = 0;
+    bool brw = false;
 };
 // Plans per batch (process-wide, guarded by g_plan_mu), the switches, and the
 // plan of this thread's last batched call.
 static std::mutex g_plan_mu;
 static std::map<std::vector<uintptr_t>, BatchPlan> g_batch_plans;
 static int g_autotune = 1;
-static BatchPlan g_forced_plan{0, 1, false, 0};  // t > 0: used instead of the cached or tuned plan
+static BatchPlan g_forced_plan{0, 1, false, 0, false};  // t > 0: used instead of the cached or tuned plan
 static thread_local BatchPlan g_last_plan;
 
 template <typename T, typename S, typename V>
@@ -469,6 +471,9 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
     // and fewer than 2^24 columns (packed into its LDS entries)
     bool rowwave_ok = sh.S > 0 && rowwave_cpl<T>(K) > 0;
     for (int h = 0; h < count; ++h) rowwave_ok = rowwave_ok && ms[h]->n_cols < (int64_t(1) << 24);
+    // the block-row wave kernel for the class rows: real vectors, FP32 values,
+    // K = 16, 32 or 64, leading dimension K (the recursion buffers)
+    const bool brw_ok = mfma_ok && std::is_same_v<V, float> && brw_cpl<T>(K) > 0;
     // row-class lists of every matrix for the plan's matrix-core threshold
     std::vector<const vbh_matrix::RowLists*> lists(count);
     auto select_all = [&](int t) {
@@ -505,14 +510,18 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
         plan = forced_plan;
         plan.P = std::min(plan.P, 64 / sh.S);
         if (!rowwave_ok) plan.rw = 0;
+        if (!brw_ok) plan.brw = false;
     } else if (have_known) {
     } else if (sh.S) {
         select_all(kMfmaMinRows);
         plan.P = default_parts();
     }
+    if (brw_ok && (tune || plan.brw))
+        for (int h = 0; h < count; ++h) ensure_brw(ms[h], s);
     select_all(plan.t);
     sh.P = sh.S ? plan.P : 1;
     sh.rw = sh.S ? plan.rw : 0;
+    sh.brw = sh.S && plan.brw ? 1 : 0;
     BatchLayout lay = batch_layout(count, ms, lists.data(), sh, mfma_ok);
     std::vector<int>& seg = lay.seg;
     std::vector<int>& seg2 = lay.seg2;
@@ -535,6 +544,9 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
             it.G = L.seg[h + 1] - L.seg[h];
             it.g_base = L.seg[h];
             it.g_base2 = L.seg2[h];
+            it.vt = m->vt;
+            it.eoff = m->eoff;
+            it.ecol = m->ecol;
         }
     };
     std::vector<double> mu0(count, 0.0);
@@ -576,8 +588,12 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
             // run side by side
             VBH_CHECK_HIP(hipEventRecord(fork.fork, st));
             VBH_CHECK_HIP(hipStreamWaitEvent(fork.side, fork.fork, 0));
-            launch_batched_mfma<T, V>(ditems, count, L.Gtot2, K, K, int(step), al, be, 0.0, -1.0, mode,
-                                      pt + int64_t(L.Gtot1) * E, fork.side);
+            if (shp.brw)
+                launch_batched_brw<T, V>(ditems, count, L.Gtot2, K, K, int(step), al, be, 0.0, -1.0, mode,
+                                         pt + int64_t(L.Gtot1) * E, fork.side);
+            else
+                launch_batched_mfma<T, V>(ditems, count, L.Gtot2, K, K, int(step), al, be, 0.0, -1.0, mode,
+                                          pt + int64_t(L.Gtot1) * E, fork.side);
         }
         if (L.Gtot1 > 0) {
             if (shp.rw)
@@ -593,8 +609,9 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
         }
     };
     if (tune) {
-        // Candidates: matrix-core thresholds (4: the 4..16-row classes on the
-        // matrix cores, 8: 8..16 only, 17: none) times slot partitions around
+        // Candidates: class thresholds (4: the 4..16-row classes, 8: 8..16
+        // only, 17: none) with the class rows on the matrix cores, or (encoded
+        // as -4, -8) on the block-row wave kernel, times slot partitions around
         // the default.  Each is timed on recursion step 1 (reads buffers 0, 1,
         // writes buffer 2; buffer 1 is cleared first), which the real run
         // overwrites; its partials go to a scratch of their own.
@@ -603,12 +620,17 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
             VBH_CHECK_HIP(hipMemsetAsync(items[h].buf[1], 0, vb, s));
         }
         std::vector<int> ts = mfma_ok ? std::vector<int>{4, 8, 17} : std::vector<int>{plan.t};
+        if (brw_ok) {
+            ts.push_back(-4);
+            ts.push_back(-8);
+        }
         const int pmax = 64 / sh.S < 16 ? 64 / sh.S : 16;
         float best = 0.f;
         hipEvent_t e0, e1;
         VBH_CHECK_HIP(hipEventCreate(&e0));
         VBH_CHECK_HIP(hipEventCreate(&e1));
-        for (int t : ts) {
+        for (int ts_ : ts) {
+            const int t = ts_ < 0 ? -ts_ : ts_;
             select_all(t);
             const int p0 = default_parts();
             // lane family: P around the row-count rule (rw = 0); row-wave
@@ -621,6 +643,7 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
                 LaneShape shc = sh;
                 shc.P = P > 0 ? P : 1;
                 shc.rw = P < 0 ? -P : 0;
+                shc.brw = ts_ < 0 ? 1 : 0;
                 BatchLayout Lc = batch_layout(count, ms, lists.data(), shc, mfma_ok);
                 fill_items(Lc);
                 Item* titems = (Item*)scratch(sizeof(Item) * count, 8);
@@ -638,6 +661,7 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
                     plan.t = t;
                     plan.P = shc.P;
                     plan.rw = shc.rw;
+                    plan.brw = shc.brw != 0;
                     plan.tuned = true;
                 }
             }
@@ -651,6 +675,7 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
         select_all(plan.t);
         sh.P = plan.P;
         sh.rw = plan.rw;
+        sh.brw = plan.brw ? 1 : 0;
         lay = batch_layout(count, ms, lists.data(), sh, mfma_ok);
         Gtot1 = lay.Gtot1;
         Gtot2 = lay.Gtot2;
@@ -703,7 +728,8 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
             GraphCache& cache = sst.graph;
             std::vector<uintptr_t> key = {uintptr_t(ditems), uintptr_t(vecs), uintptr_t(partials), uintptr_t(count),
                                           uintptr_t(K), uintptr_t(nsteps), uintptr_t(mode), uintptr_t(sizeof(T)),
-                                          uintptr_t(Gtot), uintptr_t(Gtot1), uintptr_t(sh.P), uintptr_t(sh.rw)};
+                                          uintptr_t(Gtot), uintptr_t(Gtot1), uintptr_t(sh.P), uintptr_t(sh.rw),
+                                          uintptr_t(sh.brw)};
             double ab[4] = {a1, b1, a2, b2};
             for (double v : ab) {
                 uint64_t bits;
@@ -962,12 +988,15 @@ VBH_API int vbh_set_autotune(int on) {
 VBH_API int vbh_set_batch_plan(int threshold, int parts) {
     return guard([&] {
         const int mag = parts < 0 ? -parts : parts;
-        VBH_REQUIRE(threshold == 0 || (threshold >= 1 && threshold <= 17 && mag >= 1 && mag <= 64 &&
+        const int tmag = threshold < 0 ? -threshold : threshold;
+        VBH_REQUIRE(threshold == 0 || (tmag >= 1 && tmag <= 17 && threshold != -17 && mag >= 1 && mag <= 64 &&
                                        (mag & (mag - 1)) == 0),
-                    "threshold must be 0 (release) or 1..17, with parts a power of two in 1..64 (lane family) "
-                    "or its negative (row-wave kernel, rows per work-group)");
+                    "threshold must be 0 (release), 1..17 (class rows on the matrix cores) or -16..-1 (class rows "
+                    "on the block-row wave kernel), with parts a power of two in 1..64 (lane family) or its "
+                    "negative (row-wave kernel, rows per work-group)");
         std::lock_guard<std::mutex> lk(g_plan_mu);
-        g_forced_plan.t = threshold;
+        g_forced_plan.t = tmag;
+        g_forced_plan.brw = threshold < 0;
         g_forced_plan.P = parts > 0 ? parts : 1;
         g_forced_plan.rw = parts < 0 ? -parts : 0;
         g_forced_plan.tuned = false;
@@ -977,7 +1006,7 @@ VBH_API int vbh_set_batch_plan(int threshold, int parts) {
 VBH_API int vbh_kpm_last_plan(int* threshold, int* parts, int* tuned) {
     return guard([&] {
         VBH_REQUIRE(threshold && parts && tuned, "null argument");
-        *threshold = g_last_plan.t;
+        *threshold = g_last_plan.brw ? -g_last_plan.t : g_last_plan.t;
         *parts = g_last_plan.rw ? -g_last_plan.rw : g_last_plan.P;
         *tuned = g_last_plan.tuned ? 1 : 0;
     });

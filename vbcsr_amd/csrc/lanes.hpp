@@ -621,6 +621,175 @@ __global__ __launch_bounds__(kThreads) void k_batched_rowwave(const BatchItem<T,
     write_partials(1, lane, wave, d1, d2, partials + int64_t(gg) * 2);
 }
 
+// ------------------------------------------------------------ block-row wave family
+// The matrix-core class rows of a batched step on the vector ALUs, one block
+// row per wavefront at a time.  The wave's four 16-lane rows are four
+// partitions; lane q of a partition holds columns CPL*q .. CPL*q+CPL-1 of the
+// K = 16*CPL right-hand sides for every row of the block row.  The block
+// row's stored scalar columns ("entries": block by block, column by column)
+// go round-robin to the partitions.  For entry e a partition loads its X row
+// (the 16 lanes read the whole row, contiguous) and column e of the block row
+// from the matrix's column-major FP32 copy, where the r values of a column
+// are contiguous (lane a < r takes row a); row a's value then reaches the 16
+// lanes by a DPP row broadcast folded into the FMA (v_fmac_f64 with
+// row_newbcast:a).  Every product is a useful one: no tile padding, and each
+// X row is read once per stored column, not once per row of the block.  The
+// row count is a compile-time constant of the loop (one instance per
+// r = 1..16).  At the end of a row the partitions reduce-scatter their sums
+// (partition p keeps rows a = p mod 4) and write them through the fused
+// epilogue.
+
+// acc[j] += (a of lane A_ of this 16-lane row) * x[j], j < CPL, as one asm
+// block: v_fmac_f64 with a DPP row broadcast of its first source.  The
+// compiler does not see inside the block, so its hazard recognizer cannot
+// pad for it: a VALU write of any VGPR that a DPP instruction reads needs 2
+// wait states before it, which the leading s_nop 1 provides whatever the
+// scheduler puts before the block (the FMAs inside write only their own
+// accumulators, which no later instruction of the block reads).
+template <int A_, int CPL>
+__device__ __forceinline__ void fmac_rowbcast(double (&acc)[CPL], double a, const double (&x)[CPL]) {
+    if constexpr (CPL == 1) {
+        asm("s_nop 1\n\tv_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf"
+            : "+v"(acc[0])
+            : "v"(a), "v"(x[0]), "n"(A_));
+    } else if constexpr (CPL == 2) {
+        asm("s_nop 1\n\tv_fmac_f64_dpp %0, %2, %3 row_newbcast:%5 row_mask:0xf bank_mask:0xf\n\t"
+            "v_fmac_f64_dpp %1, %2, %4 row_newbcast:%5 row_mask:0xf bank_mask:0xf"
+            : "+v"(acc[0]), "+v"(acc[1])
+            : "v"(a), "v"(x[0]), "v"(x[1]), "n"(A_));
+    } else {
+        static_assert(CPL == 4, "block-row wave kernel: CPL in 1, 2, 4");
+        asm("s_nop 1\n\tv_fmac_f64_dpp %0, %4, %5 row_newbcast:%9 row_mask:0xf bank_mask:0xf\n\t"
+            "v_fmac_f64_dpp %1, %4, %6 row_newbcast:%9 row_mask:0xf bank_mask:0xf\n\t"
+            "v_fmac_f64_dpp %2, %4, %7 row_newbcast:%9 row_mask:0xf bank_mask:0xf\n\t"
+            "v_fmac_f64_dpp %3, %4, %8 row_newbcast:%9 row_mask:0xf bank_mask:0xf"
+            : "+v"(acc[0]), "+v"(acc[1]), "+v"(acc[2]), "+v"(acc[3])
+            : "v"(a), "v"(x[0]), "v"(x[1]), "v"(x[2]), "v"(x[3]), "n"(A_));
+    }
+}
+template <int CPL, int R4, int... As>
+__device__ __forceinline__ void brw_fmas(double (&acc)[R4][CPL], double a, const double (&x)[CPL],
+                                         std::integer_sequence<int, As...>) {
+    (fmac_rowbcast<As, CPL>(acc[As], a, x), ...);
+}
+
+template <typename T, int R, int CPL>
+__device__ __forceinline__ void brw_row(const ApplyArgs<T, float>& g, const float* __restrict__ vt,
+                                        const int* __restrict__ ecol, int64_t eb, int ne, int64_t vrow, int64_t row0,
+                                        int64_t x0r, int lane, double& d1, double& d2) {
+    constexpr int R4 = (R + 3) & ~3;  // rows padded to the reduce-scatter groups (rows >= R stay zero)
+    const int q = lane & 15, part = lane >> 4;
+    const float* vcol = vt + vrow + (q < R ? q : R - 1);  // + R * e: this lane's row of column e
+    double acc[R4][CPL];
+#pragma unroll
+    for (int a = 0; a < R4; ++a)
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) acc[a][j] = 0.0;
+    // two entries per partition per pass (8 per wave); the entry columns of
+    // the next pass are loaded one pass ahead, so a pass starts with its X and
+    // value loads.  Entries past ne repeat the last one with a zero value.
+    int xr[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int e = 4 * h + part;
+        xr[h] = ecol[eb + (e < ne ? e : ne - 1)];
+    }
+    // DPP after a write of EXEC needs 5 wait states: the loop runs with the
+    // whole wave active (the row is wave-uniform), and this pads its entry
+    asm volatile("s_nop 4");
+    for (int e0 = 0; e0 < ne; e0 += 8) {
+        double av[2], xv[2][CPL];
+        int xn[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int e = e0 + 4 * h + part;
+            const int ec = e < ne ? e : ne - 1;
+            const int en = e + 8 < ne ? e + 8 : ne - 1;
+            xn[h] = ecol[eb + en];
+            const float a32 = vcol[int64_t(R) * ec];
+            av[h] = e < ne ? double(a32) : 0.0;
+            T xs[CPL];
+            load_n<T, CPL>(g.X + int64_t(xr[h]) * g.ldx + CPL * q, xs);
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) xv[h][j] = double(xs[j]);
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) brw_fmas<CPL, R4>(acc, av[h], xv[h], std::make_integer_sequence<int, R>{});
+        xr[0] = xn[0];
+        xr[1] = xn[1];
+    }
+    // reduce-scatter over the partitions: lanes 0-31 keep rows a with bit 1
+    // clear, lanes 32-63 those with it set; then within each half the even
+    // and odd partitions split bit 0.  Partition p ends with rows 4b + p.
+    const bool hi = part >= 2, odd = part & 1;
+    double k1[R4 / 4][2][CPL];
+#pragma unroll
+    for (int b = 0; b < R4 / 4; ++b)
+#pragma unroll
+        for (int s0 = 0; s0 < 2; ++s0)
+#pragma unroll
+            for (int j = 0; j < CPL; ++j) {
+                const double keep = hi ? acc[4 * b + 2 + s0][j] : acc[4 * b + s0][j];
+                const double send = hi ? acc[4 * b + s0][j] : acc[4 * b + 2 + s0][j];
+                k1[b][s0][j] = keep + shfl_xor(send, 32);
+            }
+#pragma unroll
+    for (int b = 0; b < R4 / 4; ++b) {
+        double k2[CPL];
+#pragma unroll
+        for (int j = 0; j < CPL; ++j) {
+            const double keep = odd ? k1[b][1][j] : k1[b][0][j];
+            const double send = odd ? k1[b][0][j] : k1[b][1][j];
+            k2[j] = keep + shfl_xor(send, 16);
+        }
+        const int a = 4 * b + part;
+        if (a < R) emit_slice<T, float, CPL, CPL, true>(g, row0 + a, x0r + a, CPL * q, CPL, k2, d1, d2);
+    }
+}
+
+// Batched step over the matrix-core class rows with the block-row wave
+// kernel (precisions 1 and 2: FP32 values): work-group gg takes
+// kMfmaRowsPerGroup entries of its matrix's class-row list, one per wave.
+template <typename T, int CPL>
+__global__ __launch_bounds__(kThreads) void k_batched_brw(const BatchItem<T, float>* __restrict__ items, int nitems,
+                                                          int Gtot, int K, int ld, int step, double alpha,
+                                                          double beta, double gamma0, double gamma, int mode,
+                                                          double* __restrict__ partials) {
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int gg = xcd_chunk(Gtot);
+    const int lo = batch_item<T, float, true>(items, nitems, gg);
+    using CItem = const __attribute__((address_space(4))) BatchItem<T, float>;
+    CItem& it = ((CItem*)items)[lo];
+    ApplyArgs<T, float> a = batch_args(items, lo, K, ld, step, alpha, beta, gamma0, gamma, mode);
+    const int64_t q = int64_t(gg - it.g_base2) * kMfmaRowsPerGroup + __builtin_amdgcn_readfirstlane(wave);
+    double d1 = 0.0, d2 = 0.0;
+    if (it.row_list && q < a.p.n_big) {
+        const int4 h0 = a.p.big_desc[2 * q], h1 = a.p.big_desc[2 * q + 1];
+        const int r = __builtin_amdgcn_readfirstlane(h0.z);
+        const int64_t row0 = __builtin_amdgcn_readfirstlane(h0.w), x0r = __builtin_amdgcn_readfirstlane(h1.x);
+        const int64_t vrow = int64_t((uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(h1.z))) << 32) |
+                                     uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(h1.y))));
+        const int i = __builtin_amdgcn_readfirstlane(h1.w);
+        const int64_t eb = it.eoff[i];
+        const int ne = int(it.eoff[i + 1] - eb);
+        const float* vt = it.vt;
+        const int* ecol = it.ecol;
+        if (ne > 0) {
+            switch (r) {
+#define VBH_BRW_CASE(RR) \
+    case RR: brw_row<T, RR, CPL>(a, vt, ecol, eb, ne, vrow, row0, x0r, lane, d1, d2); break;
+                VBH_BRW_CASE(1) VBH_BRW_CASE(2) VBH_BRW_CASE(3) VBH_BRW_CASE(4) VBH_BRW_CASE(5) VBH_BRW_CASE(6)
+                VBH_BRW_CASE(7) VBH_BRW_CASE(8) VBH_BRW_CASE(9) VBH_BRW_CASE(10) VBH_BRW_CASE(11)
+                VBH_BRW_CASE(12) VBH_BRW_CASE(13) VBH_BRW_CASE(14) VBH_BRW_CASE(15) VBH_BRW_CASE(16)
+#undef VBH_BRW_CASE
+                default: break;
+            }
+        }
+    }
+    write_partials(1, lane, wave, d1, d2, partials + int64_t(gg) * 2);
+}
+
 // Operands of slot u of the current 64-slot chunk, loaded straight into the
 // MFMA layout: a[m][e] = A[li][8m + 2lk + e], x[s][t] = X[xo + b(s, lk)][NT*li + t].
 // Indices are clamped into the block and into K, so every load is
@@ -941,6 +1110,27 @@ void launch_batched_rowwave(const BatchItem<T, V>* items, int nitems, int Gtot, 
 }
 
 template <typename T, typename V>
+void launch_batched_brw(const BatchItem<T, V>* items, int nitems, int Gtot, int K, int ld, int step, double alpha,
+                        double beta, double gamma0, double gamma, int mode, double* partials, hipStream_t s) {
+    if constexpr (Sc<T>::is_complex || !std::is_same_v<V, float>) {
+        throw Error(VBH_ERR_NOTSUP, "internal: the block-row wave kernel takes real vectors and FP32 values");
+    } else {
+        VBH_REQUIRE(brw_cpl<T>(K) > 0 && ld % brw_cpl<T>(K) == 0, "internal: block-row wave kernel shape");
+        auto go = [&](auto CPL_) {
+            constexpr int CPL = decltype(CPL_)::value;
+            hipLaunchKernelGGL((k_batched_brw<T, CPL>), dim3(Gtot), dim3(kThreads), 0, s, items, nitems, Gtot, K, ld,
+                               step, alpha, beta, gamma0, gamma, mode, partials);
+        };
+        switch (K / 16) {
+            case 1: go(std::integral_constant<int, 1>{}); break;
+            case 2: go(std::integral_constant<int, 2>{}); break;
+            default: go(std::integral_constant<int, 4>{}); break;
+        }
+        VBH_CHECK_HIP(hipGetLastError());
+    }
+}
+
+template <typename T, typename V>
 void launch_batched_mfma(const BatchItem<T, V>* items, int nitems, int Gtot, int K, int ld, int step, double alpha,
                          double beta, double gamma0, double gamma, int mode, double* partials, hipStream_t s) {
     if constexpr (Sc<T>::is_complex) {
@@ -997,6 +1187,8 @@ void launch_mfma(const ApplyArgs<T, V>& a, hipStream_t s) {
                                             double, double, int, double*, hipStream_t);                        \
     template void launch_batched_rowwave<T, V>(const BatchItem<T, V>*, int, int, int, int, int, int, double,   \
                                                double, double, double, int, double*, hipStream_t);             \
+    template void launch_batched_brw<T, V>(const BatchItem<T, V>*, int, int, int, int, int, double, double,     \
+                                           double, double, int, double*, hipStream_t);                         \
     template void launch_lanes<T, V>(const ApplyArgs<T, V>&, int64_t, LaneShape, int, hipStream_t);            \
     template void launch_batched_lanes<T, V>(const BatchItem<T, V>*, int, int, int, int, LaneShape, int, double, \
                                              double, double, double, int, double*, hipStream_t);

@@ -434,7 +434,7 @@ void destroy_matrix(vbh_matrix* m) {
     if (!m) return;
     for (void* p : {(void*)m->rbs, (void*)m->cbs, (void*)m->row_ptr, (void*)m->col_ind, (void*)m->roff,
                     (void*)m->coff, (void*)m->voff, (void*)m->meta, (void*)m->row_of, m->vals, m->vals32,
-                    (void*)m->split, (void*)m->t_ptr,
+                    (void*)m->split, (void*)m->t_ptr, (void*)m->vt, (void*)m->eoff, (void*)m->ecol,
                     (void*)m->t_row, (void*)m->t_slot, (void*)m->t_split})
         dfree(p);
     for (auto& kv : m->row_lists) {
@@ -469,6 +469,85 @@ void ensure_vals32(vbh_matrix* m, hipStream_t s) {
 void values_changed(vbh_matrix* m) {
     std::lock_guard<std::mutex> lk(m->lazy_mu);
     m->vals32_valid = false;
+    m->vt_valid = false;
+}
+
+// Entry columns of block row i (one wave per row): the stored scalar columns
+// of the row in slot order, entry panel + b of a block -> its X row coff + b.
+__global__ __launch_bounds__(256) void k_entry_cols(const int* __restrict__ row_ptr, const int2* __restrict__ meta,
+                                                    const int64_t* __restrict__ eoff, int nbr, int* __restrict__ ecol) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= nbr) return;
+    int* out = ecol + eoff[i];
+    for (int k = row_ptr[i] + lane; k < row_ptr[i + 1]; k += 64) {
+        const int2 mm = meta[k];
+        const int c = mm.y & 255, panel = mm.y >> 8;
+        for (int b = 0; b < c; ++b) out[panel + b] = mm.x + b;
+    }
+}
+
+// FP32 values with every block column-major (one wave per block row): block
+// k's element (a, b) at vt[voff[k] + b * r + a].
+__global__ __launch_bounds__(256) void k_values_colmajor(const double* __restrict__ vals,
+                                                         const int64_t* __restrict__ voff, const int* __restrict__ row_ptr,
+                                                         const int* __restrict__ rbs, const int* __restrict__ cbs,
+                                                         const int* __restrict__ col_ind, int nbr, float* __restrict__ vt) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= nbr) return;
+    const int r = rbs[i];
+    for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+        const int c = cbs[col_ind[k]];
+        const int64_t base = voff[k];
+        for (int v = lane; v < r * c; v += 64) {
+            const int a = v / c, b = v - a * c;
+            vt[base + int64_t(b) * r + a] = float(vals[base + v]);
+        }
+    }
+}
+
+void ensure_brw(vbh_matrix* m, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(m->lazy_mu);
+    VBH_REQUIRE(m->dtype == VBH_F64, "internal: the block-row wave kernel takes real matrices");
+    if (m->vt_valid && m->ecol) return;
+    const unsigned grid = unsigned((m->nbr + 3) / 4);
+    if (!m->ecol) {
+        std::vector<int64_t> eo(size_t(m->nbr) + 1, 0);
+        for (int i = 0; i < m->nbr; ++i) {
+            const int64_t nv = m->h_voff[m->h_row_ptr[i + 1]] - m->h_voff[m->h_row_ptr[i]];
+            eo[i + 1] = eo[i] + (m->h_rbs[i] > 0 ? nv / m->h_rbs[i] : 0);
+        }
+        int64_t* d_eo = upload(eo, s);
+        int* d_ec = nullptr;
+        try {
+            d_ec = (int*)dalloc(sizeof(int) * size_t(std::max<int64_t>(eo[m->nbr], 1)));
+            if (m->nbr > 0) {
+                hipLaunchKernelGGL(k_entry_cols, dim3(grid), dim3(256), 0, s, m->row_ptr, m->meta, d_eo, m->nbr,
+                                   d_ec);
+                VBH_CHECK_HIP(hipGetLastError());
+            }
+            VBH_CHECK_HIP(hipStreamSynchronize(s));
+        } catch (...) {
+            dfree(d_eo);
+            dfree(d_ec);
+            throw;
+        }
+        m->eoff = d_eo;
+        m->ecol = d_ec;
+    }
+    if (!m->vt) {
+        m->vt = (float*)dalloc(sizeof(float) * size_t(m->nnz + kVecPad));
+        VBH_CHECK_HIP(hipMemsetAsync(m->vt, 0, sizeof(float) * size_t(m->nnz + kVecPad), s));
+    }
+    if (m->nbr > 0) {
+        hipLaunchKernelGGL(k_values_colmajor, dim3(grid), dim3(256), 0, s, (const double*)m->vals, m->voff,
+                           m->row_ptr, m->rbs, m->cbs, m->col_ind, m->nbr, m->vt);
+        VBH_CHECK_HIP(hipGetLastError());
+    }
+    // read by kernels on any stream: complete before it is marked valid
+    VBH_CHECK_HIP(hipStreamSynchronize(s));
+    m->vt_valid = true;
 }
 
 bool has_vals32(vbh_matrix* m) {

@@ -59,16 +59,18 @@ def set_autotune(on: bool) -> None:
 
 
 def set_batch_plan(threshold: int = 0, parts: int = 1) -> None:
-    """Pin the tile plan of later batched calls (matrix-core threshold 1..17,
-    17 = no matrix-core rows; parts > 0: lane-family slot partitions, parts < 0:
-    the row-wave kernel with -parts rows per work-group); threshold 0
-    releases it."""
+    """Pin the tile plan of later batched calls (class threshold 1..17, the
+    class rows on the matrix cores, 17 = no class rows; -16..-1: class rows
+    of -threshold..16 rows on the block-row wave kernel, FP32 values and K in
+    16, 32, 64; parts > 0: lane-family slot partitions, parts < 0: the
+    row-wave kernel with -parts rows per work-group); threshold 0 releases
+    it."""
     _core.set_batch_plan(int(threshold), int(parts))
 
 
 def last_plan() -> Tuple[int, int, bool]:
-    """(matrix-core threshold, slot partitions or -(row-wave rows per
-    work-group), tuned) of the last batched call."""
+    """(class threshold (negative: block-row wave kernel), slot partitions or
+    -(row-wave rows per work-group), tuned) of the last batched call."""
     return tuple(_core.kpm_last_plan())
 
 
