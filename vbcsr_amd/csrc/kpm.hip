@@ -611,8 +611,8 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
     if (tune) {
         // Candidates: class thresholds (4: the 4..16-row classes, 8: 8..16
         // only, 17: none) with the class rows on the matrix cores, or (encoded
-        // as -4, -8) on the block-row wave kernel, times slot partitions around
-        // the default.  Each is timed on recursion step 1 (reads buffers 0, 1,
+        // as -1 (every row), -4, -8) on the block-row wave kernel, times slot
+        // partitions around the default.  Each is timed on recursion step 1 (reads buffers 0, 1,
         // writes buffer 2; buffer 1 is cleared first), which the real run
         // overwrites; its partials go to a scratch of their own.
         for (int h = 0; h < count; ++h) {
@@ -620,10 +620,8 @@ static void kpm_batched(int count, vbh_matrix* const* ms, int K, const void* con
             VBH_CHECK_HIP(hipMemsetAsync(items[h].buf[1], 0, vb, s));
         }
         std::vector<int> ts = mfma_ok ? std::vector<int>{4, 8, 17} : std::vector<int>{plan.t};
-        if (brw_ok) {
-            ts.push_back(-4);
-            ts.push_back(-8);
-        }
+        if (brw_ok)
+            for (int t : {-1, -4, -8}) ts.push_back(t);
         const int pmax = 64 / sh.S < 16 ? 64 / sh.S : 16;
         float best = 0.f;
         hipEvent_t e0, e1;

@@ -685,38 +685,46 @@ __device__ __forceinline__ void brw_row(const ApplyArgs<T, float>& g, const floa
     for (int a = 0; a < R4; ++a)
 #pragma unroll
         for (int j = 0; j < CPL; ++j) acc[a][j] = 0.0;
-    // two entries per partition per pass (8 per wave); the entry columns of
-    // the next pass are loaded one pass ahead, so a pass starts with its X and
-    // value loads.  Entries past ne repeat the last one with a zero value.
-    int xr[2];
+    // Software pipeline over passes of U entries per partition (4U per wave;
+    // more for short rows, whose passes have few FMAs to cover the loads):
+    // while the FMAs of pass j run, the X rows and values of pass j+1 are in
+    // flight and the entry columns of pass j+2 are being read.  Entries past
+    // ne repeat the last one (in bounds) with a zero value.
+    constexpr int U = R <= 2 ? 8 : R <= 5 ? 4 : 2;
+    constexpr int W = 4 * U;  // entries per pass
+    auto ent = [&](int e) { return e < ne ? e : ne - 1; };
+    int xr[U];             // entry columns of the next pass
+    float an[U];           // the next pass's values and X rows, as loaded
+    T xsn[U][CPL];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const int e = 4 * h + part;
-        xr[h] = ecol[eb + (e < ne ? e : ne - 1)];
+    for (int h = 0; h < U; ++h) {
+        const int x0 = ecol[eb + ent(4 * h + part)];
+        xr[h] = ecol[eb + ent(W + 4 * h + part)];
+        an[h] = vcol[int64_t(R) * ent(4 * h + part)];
+        load_n<T, CPL>(g.X + int64_t(x0) * g.ldx + CPL * q, xsn[h]);
     }
     // DPP after a write of EXEC needs 5 wait states: the loop runs with the
     // whole wave active (the row is wave-uniform), and this pads its entry
     asm volatile("s_nop 4");
-    for (int e0 = 0; e0 < ne; e0 += 8) {
-        double av[2], xv[2][CPL];
-        int xn[2];
+    for (int e0 = 0; e0 < ne; e0 += W) {
+        double av[U], xv[U][CPL];
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int e = e0 + 4 * h + part;
-            const int ec = e < ne ? e : ne - 1;
-            const int en = e + 8 < ne ? e + 8 : ne - 1;
-            xn[h] = ecol[eb + en];
-            const float a32 = vcol[int64_t(R) * ec];
-            av[h] = e < ne ? double(a32) : 0.0;
-            T xs[CPL];
-            load_n<T, CPL>(g.X + int64_t(xr[h]) * g.ldx + CPL * q, xs);
+        for (int h = 0; h < U; ++h) {
+            av[h] = e0 + 4 * h + part < ne ? double(an[h]) : 0.0;
 #pragma unroll
-            for (int j = 0; j < CPL; ++j) xv[h][j] = double(xs[j]);
+            for (int j = 0; j < CPL; ++j) xv[h][j] = double(xsn[h][j]);
+        }
+        int xn[U];
+#pragma unroll
+        for (int h = 0; h < U; ++h) {
+            xn[h] = ecol[eb + ent(e0 + 2 * W + 4 * h + part)];
+            an[h] = vcol[int64_t(R) * ent(e0 + W + 4 * h + part)];
+            load_n<T, CPL>(g.X + int64_t(xr[h]) * g.ldx + CPL * q, xsn[h]);
         }
 #pragma unroll
-        for (int h = 0; h < 2; ++h) brw_fmas<CPL, R4>(acc, av[h], xv[h], std::make_integer_sequence<int, R>{});
-        xr[0] = xn[0];
-        xr[1] = xn[1];
+        for (int h = 0; h < U; ++h) brw_fmas<CPL, R4>(acc, av[h], xv[h], std::make_integer_sequence<int, R>{});
+#pragma unroll
+        for (int h = 0; h < U; ++h) xr[h] = xn[h];
     }
     // reduce-scatter over the partitions: lanes 0-31 keep rows a with bit 1
     // clear, lanes 32-63 those with it set; then within each half the even
