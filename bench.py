@@ -54,6 +54,11 @@ def parse():
                         "9-40 s per precision; 5 keep the default run short)")
     p.add_argument("--mcweeny-legs", default="fp64,fp64_hermitian,fp32,bf16",
                    help="comma list of McWeeny legs (the first one is the reference of the others' errors)")
+    p.add_argument("--batched-plan", default="",
+                   help="pin the batched KPM plan as THRESHOLD:PARTS (see kpm.set_batch_plan) instead of autotuning")
+    p.add_argument("--batched-legs", default="mixed,fp32,fp64",
+                   help="comma list of batched legs; without fp64 the speedups and DOS errors are not computed "
+                        "(for counter runs)")
     p.add_argument("--copy-calibration", action="store_true",
                    help="only run a 1 GiB device-to-device copy 5 times (scale for the HBM byte counters)")
     return p.parse_args()
@@ -519,6 +524,9 @@ def main():
             V0s.append(v)
         sync()
         res = {}
+        legs = [g for g in args.batched_legs.split(",") if g]
+        if args.batched_plan:
+            KP.set_batch_plan(*map(int, args.batched_plan.split(":")))
 
         def batched(prec):
             def run():
@@ -529,34 +537,42 @@ def main():
             res["fp64"] = np.stack([KP.kpm_moments(Hm, v, Mb, mode="plain") for Hm, v in zip(mats, V0s)])
 
         bsteps = max(2, args.steps // 5)
-        tb, plans = {}, {}
-        for prec in ("mixed", "fp32"):
+        tb, plans, step_us = {}, {}, {}
+        precs = [g for g in ("mixed", "fp32") if g in legs]
+        for prec in precs:
             tb[prec] = timed(batched(prec), bsteps, 1, sync)
             t_, p_, tuned_ = KP.last_plan()
-            plans[prec] = {"mfma_min_rows": t_, "slot_parts": p_, "autotuned": tuned_}
-        t64 = timed(plain_fp64, max(1, bsteps // 2), 1, sync)
-        errs = {}
-        for prec in ("mixed", "fp32"):
-            e = []
-            for h in range(count):
-                _, r64 = KP.dos_from_moments(res["fp64"][h])
-                _, rp = KP.dos_from_moments(res[prec][h])
-                e.append(KP.dos_error(rp, r64))
-            errs[prec] = float(max(e))
+            plans[prec] = {"class_threshold": t_, "slot_parts": p_, "autotuned": tuned_}
+            kms, steps_ = vbcsr_amd._core.kpm_last_step_time()  # the graph's time over its recursion steps
+            step_us[prec] = kms * 1e3 / max(steps_, 1)
+        if args.batched_plan:
+            KP.set_batch_plan(0)
         rows = int(sum(A.n for A in Hs))
-        out["batched"] = {"workload": f"{count} random-geometric Hamiltonians, blocks {{1,4,9,13}}, "
-                                      f"{rows} rows total, each spectrum scaled to [-0.99, 0.99] by its "
-                                      f"device-Lanczos bounds, {Mb} moments x {Kb} vectors each",
-                          "spectral_bounds_first4": [list(b) for b in bounds[:4]],
-                          "nnz_total": int(sum(A.nnz for A in Hs)),
-                          "fp64_plain_moment_vecs_per_s": count * Mb * Kb / t64}
-        for prec in ("mixed", "fp32"):
-            out["batched"][prec] = {"moment_vecs_per_s": count * Mb * Kb / tb[prec],
-                                    "speedup_vs_fp64_plain": t64 / tb[prec], "max_dos_error": errs[prec],
-                                    "plan": plans[prec]}
-            log(f"batched {prec}: plan {plans[prec]}")
-            log(f"batched {prec}: {tb[prec] * 1e3:.2f} ms, speedup {t64 / tb[prec]:.2f}x vs fp64 plain "
-                f"{t64 * 1e3:.2f} ms, max DOS err {errs[prec]:.2e}")
+        bres = {"workload": f"{count} random-geometric Hamiltonians, blocks {{1,4,9,13}}, "
+                            f"{rows} rows total, each spectrum scaled to [-0.99, 0.99] by its "
+                            f"device-Lanczos bounds, {Mb} moments x {Kb} vectors each",
+                "spectral_bounds_first4": [list(b) for b in bounds[:4]],
+                "nnz_total": int(sum(A.nnz for A in Hs)),
+                "plan_note": "class_threshold < 0: class rows on the block-row wave kernel (vector ALUs), > 0: "
+                             "on the FP64 matrix cores; slot_parts < 0: row-wave kernel rows per work-group"}
+        t64 = timed(plain_fp64, max(1, bsteps // 2), 1, sync) if "fp64" in legs else None
+        if t64 is not None:
+            bres["fp64_plain_moment_vecs_per_s"] = count * Mb * Kb / t64
+        for prec in precs:
+            r = {"moment_vecs_per_s": count * Mb * Kb / tb[prec], "us_per_step": step_us[prec], "plan": plans[prec]}
+            if t64 is not None:
+                e = []
+                for h in range(count):
+                    _, r64 = KP.dos_from_moments(res["fp64"][h])
+                    _, rp = KP.dos_from_moments(res[prec][h])
+                    e.append(KP.dos_error(rp, r64))
+                r["speedup_vs_fp64_plain"] = t64 / tb[prec]
+                r["max_dos_error"] = float(max(e))
+            bres[prec] = r
+            log(f"batched {prec}: plan {plans[prec]}, {tb[prec] * 1e3:.2f} ms ({step_us[prec]:.1f} us per step)"
+                + (f", speedup {r['speedup_vs_fp64_plain']:.2f}x vs fp64 plain {t64 * 1e3:.2f} ms, "
+                   f"max DOS err {r['max_dos_error']:.2e}" if t64 is not None else ""))
+        out["batched"] = bres
 
     if want("prop"):
         out["propagation"] = bench_propagation(args, sync, log)
