@@ -418,9 +418,21 @@ constexpr int kNumMapCols = 8192;  // LDS column map (nbc ints, dynamic) when nb
 #endif
 constexpr int kNumWaves = VBH_NUM_WAVES;  // waves per work-group, each owning every kNumWaves-th C slot (4 beats 2 and 8)
 
-template <typename T, typename CT, bool MAP>
+// FP32 value rounded to the nearest BF16 (ties to even), kept in FP32: the
+// product of two such values is exact in FP32, so FP32 FMAs of them are BF16
+// products with FP32 accumulation, as on the BF16 matrix cores.
+__device__ inline float bf16_round(float f) {
+    unsigned u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return __uint_as_float(u & 0xffff0000u);
+}
+
+// BF: BF16 products (numeric 2) from the FP32 value copies, rounded to BF16
+// as they are loaded; accumulation in FP32 as for numeric 1.
+template <typename T, typename CT, bool MAP, bool BF = false>
 __global__ __launch_bounds__(64 * kNumWaves) void k_numeric_owned(NumArgs<T, CT> a, int nbc) {
     constexpr bool NS = !std::is_same_v<T, CT>;  // narrow storage: a_c, b_c, c_c
+    static_assert(!BF || std::is_same_v<CT, float>, "BF16 products from FP32 copies");
     extern __shared__ int colmap[];  // nbc entries when MAP
     const int i = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -448,6 +460,9 @@ __global__ __launch_bounds__(64 * kNumWaves) void k_numeric_owned(NumArgs<T, CT>
             const CT* Ac = a.a_c + a.a_voff[ka];
 #pragma unroll
             for (int q = 0; q < 8; ++q) arow[q] = (ra < r && q < m) ? Ac[ra * m + q] : Sc<CT>::zero();
+            if constexpr (BF)
+#pragma unroll
+                for (int q = 0; q < 8; ++q) arow[q] = bf16_round(arow[q]);
         } else {
 #pragma unroll
             for (int q = 0; q < 8; ++q)
@@ -508,7 +523,8 @@ __global__ __launch_bounds__(64 * kNumWaves) void k_numeric_owned(NumArgs<T, CT>
                     if constexpr (NS && std::is_same_v<CT, float>) {
                         acc[u] = a.c_c[co[u] + rac * cc[u] + cbc];
                         const int nb = m * cc[u];
-                        const float ball = a.b_c[bo[u] + (lane < nb ? lane : nb - 1)];
+                        float ball = a.b_c[bo[u] + (lane < nb ? lane : nb - 1)];
+                        if constexpr (BF) ball = bf16_round(ball);
 #pragma unroll
                         for (int q = 0; q < 8; ++q) {
                             const int src = q * cc[u] + cbc;
@@ -572,11 +588,7 @@ typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
 
 __device__ inline short to_bf16(double x) {
     // round-to-nearest-even on the FP32 value
-    float f = float(x);
-    unsigned u = __float_as_uint(f);
-    const unsigned lsb = (u >> 16) & 1u;
-    u += 0x7fffu + lsb;
-    return short(u >> 16);
+    return short(__float_as_uint(bf16_round(float(x))) >> 16);
 }
 
 struct MfmaArgs {
@@ -1530,7 +1542,12 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
         } else if (Cm->nnzb > 0) {
             // the sparse kernels write C's block norms for the filter (fused)
             if (thr > 0.0) d_cnorm = tmp.own((double*)dalloc(sizeof(double) * size_t(Cm->nnzb)));
-            if (numeric == 2) {
+            // BF16 products of blocks up to 8 x 8 run on the vector ALUs (the
+            // owned kernel on BF16-rounded FP32 copies): 16 x 16 x 32 matrix-core
+            // tiles would be at most an eighth full.  Larger blocks take the
+            // matrix cores.
+            const bool bf_owned = numeric == 2 && A->max_block <= 8 && B->max_block <= 8;
+            if (numeric == 2 && !bf_owned) {
                 std::vector<int> colstart(Cm->nnzb), width(nbr);
                 for (int i = 0; i < nbr; ++i) {
                     int w = 0;
@@ -1553,9 +1570,10 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
                 VBH_CHECK_HIP(hipGetLastError());
                 VBH_CHECK_HIP(hipStreamSynchronize(s));
             } else {
-                auto launch = [&](auto* tag, auto* ctag) {
+                auto launch = [&](auto* tag, auto* ctag, auto bf_) {
                     using T = std::remove_pointer_t<decltype(tag)>;
                     using CT = std::remove_pointer_t<decltype(ctag)>;
+                    constexpr bool BF = decltype(bf_)::value;
                     NumArgs<T, CT> na{A->row_ptr, A->col_ind, A->voff, (const T*)A->vals, an, A->rbs, A->cbs,
                                       B->row_ptr, B->col_ind, B->voff, (const T*)B->vals, bn, Cm->row_ptr,
                                       Cm->col_ind, Cm->voff, Cm->cbs, (T*)Cm->vals, d_np, thr,
@@ -1573,26 +1591,28 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
                             VBH_CHECK_HIP(hipMemsetAsync(na.c_c, 0, sizeof(CT) * size_t(Cm->nnz), s));
                         }
                         if (Cm->nbc <= kNumMapCols)
-                            hipLaunchKernelGGL((k_numeric_owned<T, CT, true>), dim3(nbr), dim3(64 * kNumWaves),
+                            hipLaunchKernelGGL((k_numeric_owned<T, CT, true, BF>), dim3(nbr), dim3(64 * kNumWaves),
                                                sizeof(int) * size_t(Cm->nbc), s, na, Cm->nbc);
                         else
-                            hipLaunchKernelGGL((k_numeric_owned<T, CT, false>), dim3(nbr), dim3(64 * kNumWaves), 0, s, na,
-                                               Cm->nbc);
+                            hipLaunchKernelGGL((k_numeric_owned<T, CT, false, BF>), dim3(nbr), dim3(64 * kNumWaves), 0,
+                                               s, na, Cm->nbc);
                         if constexpr (narrow_products) {
                             VBH_CHECK_HIP(hipGetLastError());
                             VBH_CHECK_HIP(hipStreamSynchronize(s));  // before c_c is freed
                         }
-                    } else {
+                    } else if constexpr (!BF) {
                         hipLaunchKernelGGL((k_numeric<T, CT>), dim3(nbr), dim3(256), 0, s, na);
                     }
                     VBH_CHECK_HIP(hipGetLastError());
                 };
+                using no_bf = std::false_type;
                 if (A->dtype == VBH_F64) {
-                    if (numeric == 0) launch((double*)nullptr, (double*)nullptr);
-                    else launch((double*)nullptr, (float*)nullptr);
+                    if (numeric == 0) launch((double*)nullptr, (double*)nullptr, no_bf{});
+                    else if (numeric == 1) launch((double*)nullptr, (float*)nullptr, no_bf{});
+                    else launch((double*)nullptr, (float*)nullptr, std::true_type{});
                 } else {
-                    if (numeric == 0) launch((c128*)nullptr, (c128*)nullptr);
-                    else launch((c128*)nullptr, (c64*)nullptr);
+                    if (numeric == 0) launch((c128*)nullptr, (c128*)nullptr, no_bf{});
+                    else launch((c128*)nullptr, (c64*)nullptr, no_bf{});
                 }
             }
         }
