@@ -193,3 +193,45 @@ def test_block_row_wave_plans(kpm_api, K, threshold):
         A = W.BlockCSR(Hs[h].bs, Hs[h].row_ptr, Hs[h].col_ind, 0.5 * Hs[h].vals)
         ref = O.kpm_moments(O.Mat.of(rounded32(A)), np.asarray(V0s[h].to_numpy()), M, mode=1)
         assert relerr(mu[h], ref) < 1e-11, h
+
+
+def _drop_block_row(A, i):
+    """A with every block of block row i and block column i removed (a
+    symmetric matrix with an empty block row)."""
+    from vbcsr_amd.workloads import BlockCSR
+
+    rp, ci, bs = A.row_ptr, A.col_ind, A.bs
+    voff = np.zeros(len(ci) + 1, np.int64)
+    for r in range(A.nb):
+        for k in range(rp[r], rp[r + 1]):
+            voff[k + 1] = voff[k] + bs[r] * bs[ci[k]]
+    new_rp, new_ci, vals = [0], [], []
+    for r in range(A.nb):
+        for k in range(rp[r], rp[r + 1]):
+            if r != i and ci[k] != i:
+                new_ci.append(ci[k])
+                vals.append(A.vals[voff[k]:voff[k + 1]])
+        new_rp.append(len(new_ci))
+    return BlockCSR(bs, np.asarray(new_rp, np.int32), np.asarray(new_ci, np.int32), np.concatenate(vals))
+
+
+@pytest.mark.parametrize("threshold", [-1, -4])
+def test_block_row_wave_empty_block_row(kpm_api, threshold):
+    """A class row without stored blocks still gets Y = beta X + gamma Z."""
+    from vbcsr_amd import workloads as W
+
+    Hs = W.batched_hamiltonians(count=2, nb_lo=40, nb_hi=80, neighbours=8.0, seed=47, sizes=(4, 9, 13))
+    Hs[1] = _drop_block_row(Hs[1], 7)
+    mats = [mk(A) for A in Hs]
+    V0s = []
+    for h, H in enumerate(mats):
+        v = H.create_multivector(32)
+        v.set_random(60 + h, normalize=False)
+        V0s.append(v)
+    Kapi = kpm_api
+    Kapi.set_batch_plan(threshold, 1)
+    mu = Kapi.kpm_moments_batched(mats, V0s, 16, mode="two", precision="mixed")
+    assert Kapi.last_plan()[0] == threshold
+    for h in range(2):
+        ref = O.kpm_moments(O.Mat.of(rounded32(Hs[h])), np.asarray(V0s[h].to_numpy()), 16, mode=1)
+        assert relerr(mu[h], ref) < 1e-11, h
