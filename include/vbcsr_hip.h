@@ -296,9 +296,14 @@ VBH_API int vbh_matrix_axpby(vbh_matrix_t self, const double* alpha, vbh_matrix_
  * filter block_csr.hpp:1399).  A: nbr x nbk blocks, B: nbk x nbc; B's block
  * rows must be A's local columns.  numeric: 0 = native FP64/complex128,
  * 1 = FP32 block products (from FP32 copies of A's and B's values, kept on
- * the device as for precision-1 applies), 2 = BF16 block products on MFMA
- * (FP32 accumulate; real matrices only).  Result in *out.  stats (optional, 4 int64): symbolic
- * pairs, numeric pairs, C blocks before filter, after. */
+ * the device as for precision-1 applies), 2 = BF16 block products with FP32
+ * accumulation (real matrices only): the BF16 matrix cores for products
+ * whose C fills >= 10 % of its block pattern (one dense GEMM, every
+ * candidate pair multiplied) and for blocks over 8 x 8, else BF16-rounded
+ * FP32 copies on the vector ALUs.  The threshold filter uses C's block norms
+ * from the numeric kernels.  Result in *out.  stats (optional, 4 int64):
+ * symbolic pairs, numeric pairs (for the dense BF16 GEMM: the symbolic
+ * pairs), C blocks before filter, after. */
 VBH_API int vbh_spgemm(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric, vbh_matrix_t* out,
                        int64_t* stats, void* stream);
 /* Upper block triangle of the thresholded SpGEMM (block columns >= the
