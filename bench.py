@@ -226,6 +226,23 @@ def bench_propagation(args, sync, log):
                 worst = max(worst, abs(fd - an) / max(abs(an), 1e-3 * np.abs(g64).max()))
     res["grad_fd_max_rel_err_fp64"] = worst
     log(f"propagation gradient: FD max rel err {worst:.2e}, fp32 vs fp64 {res['grad_rel_err_fp32_vs_fp64']:.2e}")
+    if not args.no_cpu and not args.quick:
+        import oracle
+
+        if oracle.ref_available():
+            # the reference's propagation is a Python Chebyshev series over
+            # BlockSpMat::mult (vbcsr/utils.py:71-126); its SpMVs alone, timed
+            # on the host, bound its forward pass from below
+            R = oracle.RefMat(oracle.Mat.of(A))
+            t_mult = R.time_mult(np.asarray(psi0, np.complex128).reshape(-1), 50)
+            est = t_mult * nsteps * (nterms - 1)
+            res["cpu_reference"] = {
+                "spmv_ms": t_mult * 1e3, "forward_spmv_seconds_estimate": est,
+                "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "reference",
+                "sample": "vbcsr C++ core BlockSpMat::mult (oracle/_ref), complex128, mean of 50 calls; times "
+                          f"{nsteps} x {nterms - 1} SpMVs of the forward (vector ops not counted)",
+                "gpu_fp64_forward_speedup_over_estimate": est / (res["fp64"]["forward_ms"] * 1e-3)}
+            log(f"propagation, reference CPU SpMV {t_mult * 1e3:.3f} ms -> forward >= {est:.2f} s")
     return res
 
 
@@ -572,6 +589,32 @@ def main():
             log(f"batched {prec}: plan {plans[prec]}, {tb[prec] * 1e3:.2f} ms ({step_us[prec]:.1f} us per step)"
                 + (f", speedup {r['speedup_vs_fp64_plain']:.2f}x vs fp64 plain {t64 * 1e3:.2f} ms, "
                    f"max DOS err {r['max_dos_error']:.2e}" if t64 is not None else ""))
+        if not args.no_cpu and not args.quick:
+            import oracle
+
+            if oracle.ref_available():
+                # the plain FP64 ex08 composition of the reference core on every
+                # 8th Hamiltonian (device-scaled values, same vectors), scaled
+                # to the batch by stored values
+                from vbcsr_amd.workloads import BlockCSR
+
+                secs, nnz_s = 0.0, 0
+                for h in range(0, count, 8):
+                    A = Hs[h]
+                    vals = np.asarray(mats[h].get_values())
+                    R = oracle.RefMat(oracle.Mat.of(BlockCSR(A.bs, A.row_ptr, A.col_ind, vals)))
+                    _, t_ = R.kpm_moments(np.asarray(V0s[h].to_numpy()), Mb)
+                    secs += t_
+                    nnz_s += A.nnz
+                est = secs * bres["nnz_total"] / nnz_s
+                bres["cpu_reference"] = {
+                    "sample_hamiltonians": len(range(0, count, 8)), "sample_seconds": secs,
+                    "batch_seconds_estimate": est,
+                    "threads": int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)), "kind": "reference",
+                    "sample": f"vbcsr C++ core (oracle/_ref), ex08 composition, FP64, {Mb} moments x {Kb} vectors on "
+                              "every 8th Hamiltonian, scaled to the batch by stored values",
+                    "gpu_mixed_speedup": est / tb["mixed"] if "mixed" in tb else None}
+                log(f"batched, reference CPU: {secs:.2f} s on the sample -> {est:.1f} s for the batch")
         out["batched"] = bres
 
     if want("prop"):
