@@ -5,12 +5,15 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstdio>
+#include <exception>
 #include <map>
 #include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/vbcsr_hip.h"
@@ -281,6 +284,39 @@ struct vbh_matrix {
 };
 
 namespace vbh {
+// Host work over the block rows [0, nbr) of a CSR pattern (row_ptr) in
+// contiguous row ranges of about equal slot counts, on up to 16 threads
+// (serial below ~2^18 slots, where threads do not pay): fn(r0, r1) per
+// range.  The first exception of any range is rethrown.
+template <typename F>
+void rows_parallel(int nbr, const int* row_ptr, F&& fn) {
+    const int64_t nnzb = nbr > 0 ? row_ptr[nbr] : 0;
+    const int hw = int(std::max(1u, std::thread::hardware_concurrency()));
+    const int nt = int(std::min<int64_t>(std::min(hw, 16), std::max<int64_t>(1, nnzb >> 18)));
+    if (nt <= 1 || nbr < 2 * nt) {
+        fn(0, nbr);
+        return;
+    }
+    std::vector<int> cut(nt + 1, nbr);
+    cut[0] = 0;
+    for (int t = 1; t < nt; ++t)
+        cut[t] = int(std::upper_bound(row_ptr, row_ptr + nbr + 1, int(nnzb * t / nt)) - row_ptr) - 1;
+    std::exception_ptr err;
+    std::mutex mu;
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            try {
+                if (cut[t] < cut[t + 1]) fn(cut[t], cut[t + 1]);
+            } catch (...) {
+                std::lock_guard<std::mutex> lk(mu);
+                if (!err) err = std::current_exception();
+            }
+        });
+    for (auto& x : th) x.join();
+    if (err) std::rethrow_exception(err);
+}
+
 size_t dtype_size(int dtype);
 void* dalloc(size_t bytes);
 void dfree(void* p);

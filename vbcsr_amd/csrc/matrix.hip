@@ -360,25 +360,48 @@ vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const
         VBH_REQUIRE(mx <= 255, "block sizes above 255 are not supported");
         m->h_voff.assign(m->nnzb + 1, 0);
         std::vector<int2> meta(m->nnzb);
-        std::vector<int64_t> work(nbr + 1, 0);
+        std::vector<int64_t> work(nbr + 1, 0), rowsz(nbr + 1, 0);
         std::vector<char> cls(size_t(mx + 1) * (mx + 1), 0);
-        for (int i = 0; i < nbr; ++i) {
-            int64_t w = 64;
-            int panel = 0;
-            for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
-                const int j = col_ind[k];
-                VBH_REQUIRE(j >= 0 && j < nbc, "column index out of range");
-                if (k > row_ptr[i]) VBH_REQUIRE(col_ind[k - 1] < j, "columns must be strictly ascending per row");
-                const int64_t sz = int64_t(rbs[i]) * cbs[j];
-                m->h_voff[k + 1] = m->h_voff[k] + sz;
-                meta[k] = make_int2(coff[j], (panel << 8) | cbs[j]);
-                panel += cbs[j];
-                VBH_REQUIRE(panel < (1 << 23), "block row wider than 2^23 scalar columns");
-                w += sz + 16;
-                cls[size_t(rbs[i]) * (mx + 1) + cbs[j]] = 1;
+        std::mutex cls_mu;
+        // per row (in parallel row ranges): checks, slot metadata, the row's
+        // value count and work weight; then the row prefixes; then the value
+        // offsets of every slot
+        rows_parallel(nbr, row_ptr, [&](int r0, int r1) {
+            std::vector<char> mine(cls.size(), 0);
+            for (int i = r0; i < r1; ++i) {
+                int64_t w = 64, sz_row = 0;
+                int panel = 0;
+                for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+                    const int j = col_ind[k];
+                    VBH_REQUIRE(j >= 0 && j < nbc, "column index out of range");
+                    if (k > row_ptr[i]) VBH_REQUIRE(col_ind[k - 1] < j, "columns must be strictly ascending per row");
+                    const int64_t sz = int64_t(rbs[i]) * cbs[j];
+                    meta[k] = make_int2(coff[j], (panel << 8) | cbs[j]);
+                    panel += cbs[j];
+                    VBH_REQUIRE(panel < (1 << 23), "block row wider than 2^23 scalar columns");
+                    sz_row += sz;
+                    w += sz + 16;
+                    mine[size_t(rbs[i]) * (mx + 1) + cbs[j]] = 1;
+                }
+                rowsz[i + 1] = sz_row;
+                work[i + 1] = w;
             }
-            work[i + 1] = work[i] + w;
+            std::lock_guard<std::mutex> lk(cls_mu);
+            for (size_t c = 0; c < cls.size(); ++c) cls[c] |= mine[c];
+        });
+        for (int i = 0; i < nbr; ++i) {
+            rowsz[i + 1] += rowsz[i];
+            work[i + 1] += work[i];
         }
+        rows_parallel(nbr, row_ptr, [&](int r0, int r1) {
+            for (int i = r0; i < r1; ++i) {
+                int64_t v = rowsz[i];
+                for (int k = row_ptr[i]; k < row_ptr[i + 1]; ++k) {
+                    v += int64_t(rbs[i]) * cbs[col_ind[k]];
+                    m->h_voff[k + 1] = v;
+                }
+            }
+        });
         m->n_classes = int(std::count(cls.begin(), cls.end(), 1));
         m->nnz = m->h_voff[m->nnzb];
         // work-balanced chunks of rows, G a multiple of 8 (one run per XCD)
@@ -411,6 +434,9 @@ vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const
             for (int i = 0; i < nbr; ++i)
                 for (int t = roff[i]; t < roff[i + 1]; ++t) row_of[t] = i;
             m->row_of = upload(row_of, s);
+            // the pageable-memory uploads read the host arrays until they
+            // complete: finish them before the locals go out of scope
+            VBH_CHECK_HIP(hipStreamSynchronize(s));
         }
         m->h_roff = roff;
         m->h_coff = coff;
@@ -1052,20 +1078,40 @@ VBH_API int vbh_matrix_axpby(vbh_matrix_t self, const double* alpha, vbh_matrix_
             VBH_REQUIRE(a == 0 || b == 0 || a == b, "axpby: column block sizes differ");
             cbs[j] = a ? a : b;
         }
+        // two passes over row ranges in parallel: merged lengths, then the
+        // merged columns with the source slot of each side (-1: absent)
         std::vector<int> rp(self->nbr + 1, 0), ci, ys, xs;
-        for (int i = 0; i < self->nbr; ++i) {
-            int ky = self->h_row_ptr[i], kx = X->h_row_ptr[i];
+        auto merge_row = [&](int i, int* out_c, int* out_y, int* out_x) {
+            int ky = self->h_row_ptr[i], kx = X->h_row_ptr[i], n = 0;
             const int ey = self->h_row_ptr[i + 1], ex = X->h_row_ptr[i + 1];
             while (ky < ey || kx < ex) {
                 const int cy = ky < ey ? self->h_col_ind[ky] : INT32_MAX;
                 const int cx = kx < ex ? X->h_col_ind[kx] : INT32_MAX;
                 const int col = std::min(cy, cx);
-                ci.push_back(col);
-                ys.push_back(cy == col ? ky++ : -1);
-                xs.push_back(cx == col ? kx++ : -1);
+                if (out_c) {
+                    out_c[n] = col;
+                    out_y[n] = cy == col ? ky : -1;
+                    out_x[n] = cx == col ? kx : -1;
+                }
+                if (cy == col) ++ky;
+                if (cx == col) ++kx;
+                ++n;
             }
-            rp[i + 1] = int(ci.size());
+            return n;
+        };
+        rows_parallel(self->nbr, self->h_row_ptr.data(), [&](int r0, int r1) {
+            for (int i = r0; i < r1; ++i) rp[i + 1] = merge_row(i, nullptr, nullptr, nullptr);
+        });
+        for (int i = 0; i < self->nbr; ++i) {
+            VBH_REQUIRE(int64_t(rp[i]) + rp[i + 1] <= int64_t(INT32_MAX), "axpby: union exceeds 2^31 blocks");
+            rp[i + 1] += rp[i];
         }
+        ci.resize(size_t(rp[self->nbr]));
+        ys.resize(ci.size());
+        xs.resize(ci.size());
+        rows_parallel(self->nbr, rp.data(), [&](int r0, int r1) {
+            for (int i = r0; i < r1; ++i) merge_row(i, ci.data() + rp[i], ys.data() + rp[i], xs.data() + rp[i]);
+        });
         MatrixGuard g(build_matrix(self->nbr, nbc, self->h_rbs.data(), cbs.data(), rp.data(), ci.data(),
                                    self->dtype, s));
         vbh_matrix* c = g.m;

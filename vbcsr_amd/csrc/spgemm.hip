@@ -1239,20 +1239,23 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
         // ---- symbolic sizing (host)
         std::vector<int64_t> bound(nbr, 0), out_off(nbr + 1, 0);
         std::vector<int> tsize(nbr, 1);
-        for (int i = 0; i < nbr; ++i) {
-            int64_t b = 0;
-            for (int ka = A->h_row_ptr[i]; ka < A->h_row_ptr[i + 1]; ++ka) {
-                const int k = A->h_col_ind[ka];
-                b += B->h_row_ptr[k + 1] - B->h_row_ptr[k];
+        rows_parallel(nbr, A->h_row_ptr.data(), [&](int r0, int r1) {
+            for (int i = r0; i < r1; ++i) {
+                int64_t b = 0;
+                for (int ka = A->h_row_ptr[i]; ka < A->h_row_ptr[i + 1]; ++ka) {
+                    const int k = A->h_col_ind[ka];
+                    b += B->h_row_ptr[k + 1] - B->h_row_ptr[k];
+                }
+                bound[i] = b;
+                const int64_t cap = std::min<int64_t>(b, B->nbc);
+                out_off[i + 1] = cap;
+                int64_t S = 1;
+                while (S < 2 * cap + 1) S <<= 1;
+                VBH_REQUIRE(S <= (int64_t(1) << 30), "symbolic hash table too large");
+                tsize[i] = int(S);
             }
-            bound[i] = b;
-            const int64_t cap = std::min<int64_t>(b, B->nbc);
-            out_off[i + 1] = out_off[i] + cap;
-            int64_t S = 1;
-            while (S < 2 * cap + 1) S <<= 1;
-            VBH_REQUIRE(S <= (int64_t(1) << 30), "symbolic hash table too large");
-            tsize[i] = int(S);
-        }
+        });
+        for (int i = 0; i < nbr; ++i) out_off[i + 1] += out_off[i];
         int* d_tsize = tmp.own(upload(tsize, s));
         int64_t* d_out_off = tmp.own(upload(out_off, s));
         int* d_out_cols = tmp.own((int*)dalloc(sizeof(int) * std::max<int64_t>(out_off[nbr], 1)));
@@ -1393,9 +1396,11 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
             VBH_CHECK_HIP(hipMemcpyAsync(cci.data(), d_cci, sizeof(int) * size_t(total), hipMemcpyDeviceToHost, s));
             VBH_CHECK_HIP(hipStreamSynchronize(s));
         }
-        for (int i = 0; i < nbr; ++i)
-            if (!std::is_sorted(cci.begin() + crp[i], cci.begin() + crp[i + 1]))
-                std::sort(cci.begin() + crp[i], cci.begin() + crp[i + 1]);
+        rows_parallel(nbr, crp.data(), [&](int r0, int r1) {
+            for (int i = r0; i < r1; ++i)
+                if (!std::is_sorted(cci.begin() + crp[i], cci.begin() + crp[i + 1]))
+                    std::sort(cci.begin() + crp[i], cci.begin() + crp[i + 1]);
+        });
         MatrixGuard C(build_matrix(nbr, B->nbc, A->h_rbs.data(), B->h_cbs.data(), crp.data(), cci.data(), A->dtype, s));
         unsigned long long* d_np = tmp.own((unsigned long long*)dalloc(sizeof(unsigned long long)));
         vbh_matrix* Cm = C.m;
