@@ -31,6 +31,7 @@
 #include <random>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <unordered_map>
 #include <vector>
 
@@ -404,10 +405,44 @@ struct Mat {
     }
     void compute_voff() {
         const int n = graph->n_owned();
-        voff.assign(graph->adj_ind.size() + 1, 0);
-        for (int i = 0; i < n; ++i)
-            for (int k = graph->adj_ptr[i]; k < graph->adj_ptr[i + 1]; ++k)
-                voff[k + 1] = voff[k] + int64_t(graph->block_sizes[i]) * graph->block_sizes[graph->adj_ind[k]];
+        const std::vector<int>& rp = graph->adj_ptr;
+        const std::vector<int>& ci = graph->adj_ind;
+        const std::vector<int>& bs = graph->block_sizes;
+        voff.assign(ci.size() + 1, 0);
+        // per-row value counts, their prefix, then every slot's offset; the
+        // row passes run on up to 16 host threads for large patterns
+        std::vector<int64_t> rowv(size_t(n) + 1, 0);
+        auto rows = [&](auto&& fn) {
+            const int nt = int(std::min<int64_t>(std::min<unsigned>(16u, std::max(1u, std::thread::hardware_concurrency())),
+                                                 std::max<int64_t>(1, int64_t(ci.size()) >> 18)));
+            if (nt <= 1 || n < 2 * nt) {
+                fn(0, n);
+                return;
+            }
+            std::vector<std::thread> th;
+            for (int t = 0; t < nt; ++t) {
+                const int r0 = int(int64_t(n) * t / nt), r1 = int(int64_t(n) * (t + 1) / nt);
+                th.emplace_back([&fn, r0, r1] { fn(r0, r1); });
+            }
+            for (auto& x : th) x.join();
+        };
+        rows([&](int r0, int r1) {
+            for (int i = r0; i < r1; ++i) {
+                int64_t v = 0;
+                for (int k = rp[i]; k < rp[i + 1]; ++k) v += int64_t(bs[i]) * bs[ci[k]];
+                rowv[i + 1] = v;
+            }
+        });
+        for (int i = 0; i < n; ++i) rowv[i + 1] += rowv[i];
+        rows([&](int r0, int r1) {
+            for (int i = r0; i < r1; ++i) {
+                int64_t v = rowv[i];
+                for (int k = rp[i]; k < rp[i + 1]; ++k) {
+                    v += int64_t(bs[i]) * bs[ci[k]];
+                    voff[k + 1] = v;
+                }
+            }
+        });
     }
     int64_t nnz() const { return voff.back(); }
     int64_t nnzb() const { return int64_t(graph->adj_ind.size()); }
@@ -580,7 +615,15 @@ struct Mat {
     GraphP graph_for(vbh_matrix_t m) const {
         vbh_matrix_info i{};
         check(vbh_matrix_info_get(m, &i), "info");
-        auto g = std::make_shared<Graph>(*graph);
+        // this matrix's distribution (owned and ghost blocks, sizes) with the
+        // new matrix's adjacency; the old adjacency is not copied
+        auto g = std::make_shared<Graph>();
+        g->rank = graph->rank;
+        g->size = graph->size;
+        g->owned_global_indices = graph->owned_global_indices;
+        g->ghost_global_indices = graph->ghost_global_indices;
+        g->block_sizes = graph->block_sizes;
+        g->g2l = graph->g2l;
         g->adj_ptr.assign(i.nbr + 1, 0);
         g->adj_ind.assign(i.nnzb, 0);
         check(vbh_matrix_get_pattern(m, g->adj_ptr.data(), g->adj_ind.data()), "pattern");
