@@ -352,7 +352,8 @@ def bench_mcweeny(args, sync, log):
                 s1 = P.spmm_stats
                 P3 = P2.spmm(P, thr, numeric=numeric)
             s2 = P2.spmm_stats
-            P = P2 * 3.0 - P3 * 2.0
+            P2.axpby(-2.0, P3, 3.0)  # P <- 3 P2 - 2 P3 in place of P2 (BlockSpMat::axpby; P2 is not needed again)
+            P = P2
             sync()
             per_iter.append(time.perf_counter() - ti)
             stats.append({"symbolic_pairs": int(s1[0] + s2[0]), "numeric_pairs": int(s1[1] + s2[1]),

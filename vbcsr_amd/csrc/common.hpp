@@ -231,9 +231,15 @@ struct vbh_matrix {
     int nbr = 0, nbc = 0;
     int64_t nnzb = 0, nnz = 0, n_rows = 0, n_cols = 0;
     int dtype = VBH_F64, kind = 2, max_block = 0, n_classes = 0, uniform_bs = 0;
-    // host pattern
-    std::vector<int> h_rbs, h_cbs, h_row_ptr, h_col_ind;
-    std::vector<int64_t> h_voff;
+    // host pattern: block sizes and row pointers always; the column indices
+    // and value offsets of a matrix whose pattern was made on the device
+    // (products, sums, filtered copies) are downloaded on first host use
+    // (host_col_ind / host_voff), under host_mu
+    std::vector<int> h_rbs, h_cbs, h_row_ptr;
+    mutable std::vector<int> h_col_ind;
+    mutable std::vector<int64_t> h_voff;
+    mutable bool h_cols_valid = true, h_voff_valid = true;
+    mutable std::mutex host_mu;
     // device pattern
     int *rbs = nullptr, *cbs = nullptr, *row_ptr = nullptr, *col_ind = nullptr;
     int *roff = nullptr, *coff = nullptr;
@@ -345,6 +351,22 @@ const vbh_matrix::RowLists& row_classes(vbh_matrix* m, int t, hipStream_t s);
 // Build a matrix handle from host pattern arrays (no value upload).
 vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const int* row_ptr,
                          const int* col_ind, int dtype, hipStream_t s);
+// Build a matrix handle from a pattern whose column indices are on the
+// device (d_col_ind, strictly ascending per row; copied, not adopted) and
+// whose row pointers and block sizes are on the host: the value offsets and
+// slot metadata are computed on the device, and the host mirror of the
+// columns and offsets is left to first use.
+vbh_matrix* build_matrix_dev(int nbr, int nbc, const int* rbs, const int* cbs, const int* row_ptr,
+                             const int* d_col_ind, int dtype, hipStream_t s);
+// Host mirror of the column indices / value offsets (downloaded on first use).
+const std::vector<int>& host_col_ind(const vbh_matrix* m, hipStream_t s);
+const std::vector<int64_t>& host_voff(const vbh_matrix* m, hipStream_t s);
+// Whether two matrices have the same pattern (rows, columns, block sizes);
+// compares the device column arrays.
+bool same_pattern(const vbh_matrix* a, const vbh_matrix* b, hipStream_t s);
+// Sort the columns of every row of a device pattern (row i: d_col_ind
+// [row_ptr[i], row_ptr[i+1]); row_ptr on the host, d_row_ptr its device copy).
+void sort_rows_dev(int nbr, const int* row_ptr, const int* d_row_ptr, int* d_col_ind, hipStream_t s);
 void destroy_matrix(vbh_matrix* m);
 // Filtered copy of m: blocks with norm < thr dropped (norm = sqrt(d_norm[k])
 // when squared); null when every block is kept.  Device buffers only.

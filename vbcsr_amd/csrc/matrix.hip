@@ -2,6 +2,7 @@
 // FP32 value copy, the adjoint plan, and the value-level matrix algebra
 // (scale, shift, block norms, filter, union axpby, trace).
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -194,6 +195,77 @@ __global__ __launch_bounds__(256) void k_union_axpby(const T* __restrict__ Y, co
         dst[db + i] = acc;
     }
 }
+// Union of the sorted columns of row i of two patterns (Y: yp/yc, X:
+// xp/xc).  Pass 0 (out_ptr null): counts[i] = the merged length.  Pass 1:
+// the merged columns from out_ptr[i] on, with the source slot of each side
+// (oy / ox, -1: absent).  One wavefront per row.  The merged position of a
+// column is (Y columns below it) + (X-only columns below it): for Y's
+// element k, (k - yb) + (X columns below, by binary search) - (Y columns
+// before k that X also holds); for an X-only element l, (Y columns below,
+// by binary search) + (X-only elements before l).  The two running counts
+// are ballot prefix sums over the wave's chunks.  A column in both rows is
+// written by Y's side.
+__global__ __launch_bounds__(256) void k_union_rows(int nbr, const int* __restrict__ yp, const int* __restrict__ yc,
+                                                    const int* __restrict__ xp, const int* __restrict__ xc,
+                                                    const int* __restrict__ out_ptr, int* __restrict__ counts,
+                                                    int* __restrict__ oc, int* __restrict__ oy, int* __restrict__ ox) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= nbr) return;
+    const uint64_t below = (uint64_t(1) << lane) - 1;
+    const int yb = yp[i], ye = yp[i + 1], xb = xp[i], xe = xp[i + 1];
+    auto lower = [](const int* c, int lo, int hi, int v) {
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (c[mid] < v) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    int common = 0;  // Y columns so far that X also holds
+    for (int k0 = yb; k0 < ye; k0 += 64) {
+        const int k = k0 + lane;
+        int c = 0, l = xb;
+        bool match = false;
+        if (k < ye) {
+            c = yc[k];
+            l = lower(xc, xb, xe, c);
+            match = l < xe && xc[l] == c;
+        }
+        const uint64_t mm = __ballot(match);
+        if (out_ptr && k < ye) {
+            const int pos = out_ptr[i] + (k - yb) + (l - xb) - (common + __popcll(mm & below));
+            oc[pos] = c;
+            oy[pos] = k;
+            ox[pos] = match ? l : -1;
+        }
+        common += __popcll(mm);
+    }
+    if (!out_ptr) {
+        if (lane == 0) counts[i] = (ye - yb) + (xe - xb) - common;
+        return;
+    }
+    int xonly = 0;  // X-only columns so far
+    for (int l0 = xb; l0 < xe; l0 += 64) {
+        const int l = l0 + lane;
+        int c = 0, k = yb;
+        bool only = false;
+        if (l < xe) {
+            c = xc[l];
+            k = lower(yc, yb, ye, c);
+            only = !(k < ye && yc[k] == c);
+        }
+        const uint64_t mo = __ballot(only);
+        if (only) {
+            const int pos = out_ptr[i] + (k - yb) + xonly + __popcll(mo & below);
+            oc[pos] = c;
+            oy[pos] = -1;
+            ox[pos] = l;
+        }
+        xonly += __popcll(mo);
+    }
+}
+
 template <typename T>
 __global__ void k_axpby_flat(const T* __restrict__ X, T* __restrict__ Y, int64_t n, T alpha, T beta) {
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -287,7 +359,8 @@ const vbh_matrix::RowLists& row_classes(vbh_matrix* m, int t, hipStream_t s) {
     for (int i = 0; i < nbr && !any_big; ++i) any_big = is_big(i);
     vbh_matrix::RowLists L;
     if (any_big) {
-        if (m->h_bfs.empty()) m->h_bfs = bfs_order(nbr, m->h_row_ptr.data(), m->h_col_ind.data());
+        if (m->h_bfs.empty()) m->h_bfs = bfs_order(nbr, m->h_row_ptr.data(), host_col_ind(m, s).data());
+        const std::vector<int64_t>& hv = host_voff(m, s);
         std::vector<int> small, big;
         for (int i : m->h_bfs) {
             if (is_big(i)) big.push_back(i);
@@ -298,7 +371,7 @@ const vbh_matrix::RowLists& row_classes(vbh_matrix* m, int t, hipStream_t s) {
         for (size_t q = 0; q < big.size(); ++q) {
             const int i = big[q];
             const int kb = m->h_row_ptr[i];
-            const uint64_t v = uint64_t(m->h_voff[kb]);
+            const uint64_t v = uint64_t(hv[kb]);
             desc[2 * q] = make_int4(kb, m->h_row_ptr[i + 1], rbs[i], m->h_roff[i]);
             desc[2 * q + 1] = make_int4(m->h_coff[i], int(uint32_t(v)), int(uint32_t(v >> 32)), i);
         }
@@ -318,6 +391,95 @@ const vbh_matrix::RowLists& row_classes(vbh_matrix* m, int t, hipStream_t s) {
     return m->row_lists.emplace(t, L).first->second;
 }
 
+// Block sizes of a new matrix (rows rbs, columns cbs): checks, extents,
+// kind, uniform size, and the scalar offsets of block rows and columns.
+static void set_block_layout(vbh_matrix* m, const int* rbs, const int* cbs, std::vector<int>& roff,
+                             std::vector<int>& coff) {
+    const int nbr = m->nbr, nbc = m->nbc;
+    m->h_rbs.assign(rbs, rbs + nbr);
+    m->h_cbs.assign(cbs, cbs + nbc);
+    int mn = 1 << 30, mx = 0;
+    for (int i = 0; i < nbr; ++i) {
+        VBH_REQUIRE(rbs[i] > 0, "block sizes must be positive");
+        mn = std::min(mn, rbs[i]);
+        mx = std::max(mx, rbs[i]);
+    }
+    for (int j = 0; j < nbc; ++j) {
+        VBH_REQUIRE(cbs[j] > 0, "block sizes must be positive");
+        mn = std::min(mn, cbs[j]);
+        mx = std::max(mx, cbs[j]);
+    }
+    VBH_REQUIRE(mx <= 255, "block sizes above 255 are not supported");
+    m->max_block = mx;
+    // kind detection: block_csr.hpp:2770-2798 (on the owned row sizes)
+    int rmn = 1 << 30, rmx = 0;
+    for (int i = 0; i < nbr; ++i) { rmn = std::min(rmn, rbs[i]); rmx = std::max(rmx, rbs[i]); }
+    if (nbr == 0) { rmn = 1; rmx = 1; }
+    m->kind = rmx <= 1 ? 0 : (rmn == rmx ? 1 : 2);
+    m->uniform_bs = (mn == mx && nbc > 0) ? mx : 0;
+    roff.assign(nbr + 1, 0);
+    coff.assign(nbc + 1, 0);
+    for (int i = 0; i < nbr; ++i) roff[i + 1] = roff[i] + rbs[i];
+    for (int j = 0; j < nbc; ++j) coff[j + 1] = coff[j] + cbs[j];
+    m->n_rows = roff[nbr];
+    m->n_cols = coff[nbc];
+}
+
+static void set_row_ptr(vbh_matrix* m, const int* row_ptr) {
+    m->h_row_ptr.assign(row_ptr, row_ptr + m->nbr + 1);
+    VBH_REQUIRE(m->h_row_ptr[0] == 0, "row_ptr must start at 0");
+    for (int i = 0; i < m->nbr; ++i) VBH_REQUIRE(row_ptr[i + 1] >= row_ptr[i], "row_ptr not monotone");
+    m->nnzb = row_ptr[m->nbr];
+}
+
+// The rest of a new matrix once its slots are laid out (col_ind, voff, meta
+// on the device): work-balanced chunks of rows from the inclusive prefix of
+// the row work weights (work[i + 1] = rows 0..i), the small device arrays,
+// and zeroed values.  Synchronises the stream, so host arrays the uploads
+// read may go out of scope afterwards.
+static void finish_matrix(vbh_matrix* m, const std::vector<int64_t>& work, const std::vector<int>& roff,
+                          const std::vector<int>& coff, hipStream_t s) {
+    const int nbr = m->nbr;
+    // work-balanced chunks of rows, G a multiple of 8 (one run per XCD)
+#ifndef VBH_MAX_GROUPS
+#define VBH_MAX_GROUPS 8192
+#endif
+    int G = int(std::min<int64_t>(std::max<int64_t>((nbr + kWaves - 1) / kWaves, 8), VBH_MAX_GROUPS));
+    G = (G + 7) & ~7;
+    std::vector<int> split(G + 1, nbr);
+    split[0] = 0;
+    const int64_t total = work[nbr];
+    int row = 0;
+    for (int g = 1; g < G; ++g) {
+        const int64_t target = total * g / G;
+        while (row < nbr && work[row] < target) ++row;
+        split[g] = row;
+    }
+    split[G] = nbr;
+    m->G = G;
+    if (!m->rbs) m->rbs = upload(m->h_rbs, s);
+    if (!m->cbs) m->cbs = upload(m->h_cbs, s);
+    if (!m->row_ptr) m->row_ptr = upload(m->h_row_ptr, s);
+    if (!m->roff) m->roff = upload(roff, s);
+    if (!m->coff) m->coff = upload(coff, s);
+    std::vector<int> row_of(std::max<int64_t>(m->n_rows, 1), 0);
+    for (int i = 0; i < nbr; ++i)
+        for (int t = roff[i]; t < roff[i + 1]; ++t) row_of[t] = i;
+    m->row_of = upload(row_of, s);
+    m->h_roff = roff;
+    m->h_coff = coff;
+    // the row-class lists (breadth-first order, matrix-core descriptors) are
+    // built by the first apply (make_args): products, filtered copies and
+    // sums that are never applied to vectors do not pay for them
+    m->split = upload(split, s);
+    // kVecPad trailing elements: the streaming kernels read whole 16-byte
+    // words, which may reach up to one word past the last value
+    m->vals = dalloc(m->elem_size() * (m->nnz + kVecPad));
+    VBH_CHECK_HIP(hipMemsetAsync(m->vals, 0, m->elem_size() * (m->nnz + kVecPad), s));
+    // the pageable-memory uploads read the host arrays until they complete
+    VBH_CHECK_HIP(hipStreamSynchronize(s));
+}
+
 vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const int* row_ptr, const int* col_ind,
                          int dtype, hipStream_t s) {
     VBH_REQUIRE(nbr >= 0 && nbc >= 0, "negative block counts");
@@ -327,37 +489,11 @@ vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const
         m->nbr = nbr;
         m->nbc = nbc;
         m->dtype = dtype;
-        m->h_rbs.assign(rbs, rbs + nbr);
-        m->h_cbs.assign(cbs, cbs + nbc);
-        m->h_row_ptr.assign(row_ptr, row_ptr + nbr + 1);
-        VBH_REQUIRE(m->h_row_ptr[0] == 0, "row_ptr must start at 0");
-        for (int i = 0; i < nbr; ++i) VBH_REQUIRE(row_ptr[i + 1] >= row_ptr[i], "row_ptr not monotone");
-        m->nnzb = row_ptr[nbr];
+        set_row_ptr(m, row_ptr);
         m->h_col_ind.assign(col_ind, col_ind + m->nnzb);
-        int mn = 1 << 30, mx = 0;
-        for (int i = 0; i < nbr; ++i) {
-            VBH_REQUIRE(rbs[i] > 0, "block sizes must be positive");
-            mn = std::min(mn, rbs[i]);
-            mx = std::max(mx, rbs[i]);
-        }
-        for (int j = 0; j < nbc; ++j) {
-            VBH_REQUIRE(cbs[j] > 0, "block sizes must be positive");
-            mn = std::min(mn, cbs[j]);
-            mx = std::max(mx, cbs[j]);
-        }
-        m->max_block = mx;
-        // kind detection: block_csr.hpp:2770-2798 (on the owned row sizes)
-        int rmn = 1 << 30, rmx = 0;
-        for (int i = 0; i < nbr; ++i) { rmn = std::min(rmn, rbs[i]); rmx = std::max(rmx, rbs[i]); }
-        if (nbr == 0) { rmn = 1; rmx = 1; }
-        m->kind = rmx <= 1 ? 0 : (rmn == rmx ? 1 : 2);
-        m->uniform_bs = (mn == mx && nbc > 0) ? mx : 0;
-        std::vector<int> roff(nbr + 1, 0), coff(nbc + 1, 0);
-        for (int i = 0; i < nbr; ++i) roff[i + 1] = roff[i] + rbs[i];
-        for (int j = 0; j < nbc; ++j) coff[j + 1] = coff[j] + cbs[j];
-        m->n_rows = roff[nbr];
-        m->n_cols = coff[nbc];
-        VBH_REQUIRE(mx <= 255, "block sizes above 255 are not supported");
+        std::vector<int> roff, coff;
+        set_block_layout(m, rbs, cbs, roff, coff);
+        const int mx = m->max_block;
         m->h_voff.assign(m->nnzb + 1, 0);
         std::vector<int2> meta(m->nnzb);
         std::vector<int64_t> work(nbr + 1, 0), rowsz(nbr + 1, 0);
@@ -404,56 +540,272 @@ vbh_matrix* build_matrix(int nbr, int nbc, const int* rbs, const int* cbs, const
         });
         m->n_classes = int(std::count(cls.begin(), cls.end(), 1));
         m->nnz = m->h_voff[m->nnzb];
-        // work-balanced chunks of rows, G a multiple of 8 (one run per XCD)
-#ifndef VBH_MAX_GROUPS
-#define VBH_MAX_GROUPS 8192
-#endif
-        int G = int(std::min<int64_t>(std::max<int64_t>((nbr + kWaves - 1) / kWaves, 8), VBH_MAX_GROUPS));
-        G = (G + 7) & ~7;
-        std::vector<int> split(G + 1, nbr);
-        split[0] = 0;
-        const int64_t total = work[nbr];
-        int row = 0;
-        for (int g = 1; g < G; ++g) {
-            const int64_t target = total * g / G;
-            while (row < nbr && work[row] < target) ++row;
-            split[g] = row;
-        }
-        split[G] = nbr;
-        m->G = G;
-        m->rbs = upload(m->h_rbs, s);
-        m->cbs = upload(m->h_cbs, s);
-        m->row_ptr = upload(m->h_row_ptr, s);
         m->col_ind = upload(m->h_col_ind, s);
-        m->roff = upload(roff, s);
-        m->coff = upload(coff, s);
         m->voff = upload(m->h_voff, s);
         m->meta = upload(meta, s);
-        {
-            std::vector<int> row_of(std::max<int64_t>(m->n_rows, 1), 0);
-            for (int i = 0; i < nbr; ++i)
-                for (int t = roff[i]; t < roff[i + 1]; ++t) row_of[t] = i;
-            m->row_of = upload(row_of, s);
-            // the pageable-memory uploads read the host arrays until they
-            // complete: finish them before the locals go out of scope
-            VBH_CHECK_HIP(hipStreamSynchronize(s));
-        }
-        m->h_roff = roff;
-        m->h_coff = coff;
-        // the row-class lists (breadth-first order, matrix-core descriptors) are
-        // built by the first apply (make_args): products, filtered copies and
-        // sums that are never applied to vectors do not pay for them
-        m->split = upload(split, s);
-        // kVecPad trailing elements: the streaming kernels read whole 16-byte
-        // words, which may reach up to one word past the last value
-        m->vals = dalloc(m->elem_size() * (m->nnz + kVecPad));
-        VBH_CHECK_HIP(hipMemsetAsync(m->vals, 0, m->elem_size() * (m->nnz + kVecPad), s));
-        VBH_CHECK_HIP(hipStreamSynchronize(s));
+        finish_matrix(m, work, roff, coff, s);
     } catch (...) {
         destroy_matrix(m);
         throw;
     }
     return m;
+}
+
+// Per block row (one wavefront each): the row's value count and work
+// weight, the (r, c) block classes present (a flag per class), and a check
+// of the columns (in range, strictly ascending, row narrower than 2^23 scalar
+// columns): bad[0] is set to 1 when a check fails.
+__global__ __launch_bounds__(256) void k_row_sizes(int nbr, int nbc, const int* __restrict__ row_ptr,
+                                                   const int* __restrict__ col_ind, const int* __restrict__ rbs,
+                                                   const int* __restrict__ cbs, int mx, int64_t* __restrict__ rowsz,
+                                                   int64_t* __restrict__ work, unsigned char* __restrict__ cls,
+                                                   int* __restrict__ bad) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= nbr) return;
+    const int r = rbs[i], kb = row_ptr[i], ke = row_ptr[i + 1];
+    int64_t width = 0;
+    bool ok = true;
+    for (int k = kb + lane; k < ke; k += 64) {
+        const int j = col_ind[k];
+        if (j < 0 || j >= nbc || (k > kb && col_ind[k - 1] >= j)) {
+            ok = false;
+            continue;
+        }
+        const int c = cbs[j];
+        width += c;
+        cls[r * (mx + 1) + c] = 1;
+    }
+    for (int d = 32; d >= 1; d >>= 1) width += __shfl_xor(width, d, 64);
+    const bool any_bad = __ballot(!ok) != 0;
+    if (lane == 0) {
+        rowsz[i] = int64_t(r) * width;
+        work[i] = 64 + int64_t(r) * width + 16 * int64_t(ke - kb);
+        if (any_bad || width >= (int64_t(1) << 23)) bad[0] = 1;
+    }
+}
+
+// Value offsets and slot metadata: voff[k + 1] = the row's first value +
+// r * (block widths of the row up to and including slot k); meta[k] =
+// (first scalar column of the block, offset of the block in the row's panel
+// << 8 | block width).  One wavefront per block row, an in-wave scan of the
+// block widths; voff[0] is set by the caller.
+__global__ __launch_bounds__(256) void k_slot_layout(int nbr, const int* __restrict__ row_ptr,
+                                                     const int* __restrict__ col_ind, const int* __restrict__ rbs,
+                                                     const int* __restrict__ cbs, const int* __restrict__ coff,
+                                                     const int64_t* __restrict__ rowstart, int64_t* __restrict__ voff,
+                                                     int2* __restrict__ meta) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= nbr) return;
+    const int r = rbs[i], ke = row_ptr[i + 1];
+    int64_t base = rowstart[i];
+    int panel = 0;
+    for (int k0 = row_ptr[i]; k0 < ke; k0 += 64) {
+        const int k = k0 + lane;
+        const int j = k < ke ? col_ind[k] : 0;
+        const int c = k < ke ? cbs[j] : 0;
+        int inc = c;
+        for (int d = 1; d < 64; d <<= 1) {
+            const int t = __shfl_up(inc, d, 64);
+            if (lane >= d) inc += t;
+        }
+        if (k < ke) {
+            voff[k + 1] = base + int64_t(r) * inc;
+            meta[k] = make_int2(coff[j], ((panel + inc - c) << 8) | c);
+        }
+        const int tot = __shfl(inc, 63, 64);
+        base += int64_t(r) * tot;
+        panel += tot;
+    }
+}
+
+vbh_matrix* build_matrix_dev(int nbr, int nbc, const int* rbs, const int* cbs, const int* row_ptr,
+                             const int* d_col_ind, int dtype, hipStream_t s) {
+    VBH_REQUIRE(nbr >= 0 && nbc >= 0, "negative block counts");
+    VBH_REQUIRE(dtype == VBH_F64 || dtype == VBH_C128, "matrix dtype must be VBH_F64 or VBH_C128");
+    auto* m = new vbh_matrix();
+    try {
+        m->nbr = nbr;
+        m->nbc = nbc;
+        m->dtype = dtype;
+        set_row_ptr(m, row_ptr);
+        std::vector<int> roff, coff;
+        set_block_layout(m, rbs, cbs, roff, coff);
+        const int mx = m->max_block;
+        m->h_cols_valid = m->h_voff_valid = false;
+        m->rbs = upload(m->h_rbs, s);
+        m->cbs = upload(m->h_cbs, s);
+        m->row_ptr = upload(m->h_row_ptr, s);
+        m->roff = upload(roff, s);
+        m->coff = upload(coff, s);
+        m->col_ind = (int*)dalloc(sizeof(int) * size_t(std::max<int64_t>(m->nnzb, 1)));
+        m->voff = (int64_t*)dalloc(sizeof(int64_t) * size_t(m->nnzb + 1));
+        m->meta = (int2*)dalloc(sizeof(int2) * size_t(std::max<int64_t>(m->nnzb, 1)));
+        VBH_CHECK_HIP(hipMemsetAsync(m->voff, 0, sizeof(int64_t), s));
+        std::vector<int64_t> work(nbr + 1, 0), rowstart(nbr + 1, 0);
+        const size_t ncls = size_t(mx + 1) * (mx + 1);
+        std::vector<unsigned char> cls(ncls, 0);
+        int bad = 0;
+        if (nbr > 0) {
+            DevList tmp;
+            int64_t* d_rowsz = tmp.own((int64_t*)dalloc(sizeof(int64_t) * size_t(nbr)));
+            int64_t* d_work = tmp.own((int64_t*)dalloc(sizeof(int64_t) * size_t(nbr)));
+            unsigned char* d_cls = tmp.own((unsigned char*)dalloc(ncls));
+            int* d_bad = tmp.own((int*)dalloc(sizeof(int)));
+            VBH_CHECK_HIP(hipMemsetAsync(d_cls, 0, ncls, s));
+            VBH_CHECK_HIP(hipMemsetAsync(d_bad, 0, sizeof(int), s));
+            if (m->nnzb > 0)
+                VBH_CHECK_HIP(hipMemcpyAsync(m->col_ind, d_col_ind, sizeof(int) * size_t(m->nnzb),
+                                             hipMemcpyDeviceToDevice, s));
+            const unsigned nb = unsigned((nbr + 3) / 4);
+            hipLaunchKernelGGL(k_row_sizes, dim3(nb), dim3(256), 0, s, nbr, nbc, m->row_ptr, m->col_ind, m->rbs,
+                               m->cbs, mx, d_rowsz, d_work, d_cls, d_bad);
+            VBH_CHECK_HIP(hipGetLastError());
+            VBH_CHECK_HIP(hipMemcpyAsync(rowstart.data() + 1, d_rowsz, sizeof(int64_t) * size_t(nbr),
+                                         hipMemcpyDeviceToHost, s));
+            VBH_CHECK_HIP(hipMemcpyAsync(work.data() + 1, d_work, sizeof(int64_t) * size_t(nbr),
+                                         hipMemcpyDeviceToHost, s));
+            VBH_CHECK_HIP(hipMemcpyAsync(cls.data(), d_cls, ncls, hipMemcpyDeviceToHost, s));
+            VBH_CHECK_HIP(hipMemcpyAsync(&bad, d_bad, sizeof(int), hipMemcpyDeviceToHost, s));
+            VBH_CHECK_HIP(hipStreamSynchronize(s));
+            VBH_REQUIRE(!bad, "device pattern: column index out of range, columns not strictly ascending, or a "
+                              "block row wider than 2^23 scalar columns");
+            for (int i = 0; i < nbr; ++i) {
+                rowstart[i + 1] += rowstart[i];
+                work[i + 1] += work[i];
+            }
+            int64_t* d_rowstart = tmp.own(upload(rowstart, s));
+            hipLaunchKernelGGL(k_slot_layout, dim3(nb), dim3(256), 0, s, nbr, m->row_ptr, m->col_ind, m->rbs,
+                               m->cbs, m->coff, d_rowstart, m->voff, m->meta);
+            VBH_CHECK_HIP(hipGetLastError());
+            VBH_CHECK_HIP(hipStreamSynchronize(s));  // before the temporaries are freed
+        }
+        m->n_classes = int(std::count(cls.begin(), cls.end(), (unsigned char)1));
+        m->nnz = rowstart[nbr];
+        finish_matrix(m, work, roff, coff, s);
+    } catch (...) {
+        destroy_matrix(m);
+        throw;
+    }
+    return m;
+}
+
+const std::vector<int>& host_col_ind(const vbh_matrix* m, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(m->host_mu);
+    if (!m->h_cols_valid) {
+        std::vector<int> h(size_t(m->nnzb));
+        if (m->nnzb > 0) {
+            VBH_CHECK_HIP(hipMemcpyAsync(h.data(), m->col_ind, sizeof(int) * h.size(), hipMemcpyDeviceToHost, s));
+            VBH_CHECK_HIP(hipStreamSynchronize(s));
+        }
+        m->h_col_ind.swap(h);
+        m->h_cols_valid = true;
+    }
+    return m->h_col_ind;
+}
+
+const std::vector<int64_t>& host_voff(const vbh_matrix* m, hipStream_t s) {
+    std::lock_guard<std::mutex> lk(m->host_mu);
+    if (!m->h_voff_valid) {
+        std::vector<int64_t> h(size_t(m->nnzb + 1));
+        VBH_CHECK_HIP(hipMemcpyAsync(h.data(), m->voff, sizeof(int64_t) * h.size(), hipMemcpyDeviceToHost, s));
+        VBH_CHECK_HIP(hipStreamSynchronize(s));
+        m->h_voff.swap(h);
+        m->h_voff_valid = true;
+    }
+    return m->h_voff;
+}
+
+// flag[0] = 1 when a[i] != b[i] for some i < n.
+__global__ void k_any_diff(const int* __restrict__ a, const int* __restrict__ b, int64_t n, int* __restrict__ flag) {
+    for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n; i += int64_t(gridDim.x) * blockDim.x)
+        if (a[i] != b[i]) flag[0] = 1;
+}
+
+bool same_pattern(const vbh_matrix* a, const vbh_matrix* b, hipStream_t s) {
+    if (a == b) return true;
+    if (a->nbr != b->nbr || a->nbc != b->nbc || a->nnzb != b->nnzb || a->h_row_ptr != b->h_row_ptr ||
+        a->h_rbs != b->h_rbs || a->h_cbs != b->h_cbs)
+        return false;
+    if (a->nnzb == 0) return true;
+    DevList tmp;
+    int* d_flag = tmp.own((int*)dalloc(sizeof(int)));
+    VBH_CHECK_HIP(hipMemsetAsync(d_flag, 0, sizeof(int), s));
+    const unsigned nb = unsigned(std::min<int64_t>((a->nnzb + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_any_diff, dim3(nb), dim3(256), 0, s, a->col_ind, b->col_ind, a->nnzb, d_flag);
+    VBH_CHECK_HIP(hipGetLastError());
+    int flag = 0;
+    VBH_CHECK_HIP(hipMemcpyAsync(&flag, d_flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    VBH_CHECK_HIP(hipStreamSynchronize(s));
+    return flag == 0;
+}
+
+// Rows of at most kSortMax columns are sorted in LDS (bitonic network over
+// the next power of two, INT_MAX padding); already sorted rows are left as
+// they are; a longer unsorted row sets long_rows[i] = 1 for the host.  One
+// work-group per row.
+constexpr int kSortMax = 4096;
+__global__ __launch_bounds__(256) void k_sort_rows(const int* __restrict__ row_ptr, int* __restrict__ col,
+                                                   int* __restrict__ long_rows) {
+    __shared__ int sv[kSortMax];
+    __shared__ int unsorted;
+    const int i = blockIdx.x;
+    const int kb = row_ptr[i], n = row_ptr[i + 1] - kb;
+    if (n < 2) return;
+    if (threadIdx.x == 0) unsorted = 0;
+    __syncthreads();
+    for (int t = threadIdx.x; t + 1 < n; t += 256)
+        if (col[kb + t] > col[kb + t + 1]) unsorted = 1;
+    __syncthreads();
+    if (!unsorted) return;
+    if (n > kSortMax) {
+        if (threadIdx.x == 0) long_rows[i] = 1;
+        return;
+    }
+    int P = 2;
+    while (P < n) P <<= 1;
+    for (int t = threadIdx.x; t < P; t += 256) sv[t] = t < n ? col[kb + t] : INT_MAX;
+    __syncthreads();
+    for (int k = 2; k <= P; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = threadIdx.x; t < P; t += 256) {
+                const int u = t ^ j;
+                if (u > t) {
+                    const int a = sv[t], b = sv[u];
+                    if ((a > b) == ((t & k) == 0)) {
+                        sv[t] = b;
+                        sv[u] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    for (int t = threadIdx.x; t < n; t += 256) col[kb + t] = sv[t];
+}
+
+void sort_rows_dev(int nbr, const int* row_ptr, const int* d_row_ptr, int* d_col_ind, hipStream_t s) {
+    if (nbr <= 0 || row_ptr[nbr] == 0) return;
+    DevList tmp;
+    int* d_long = tmp.own((int*)dalloc(sizeof(int) * size_t(nbr)));
+    VBH_CHECK_HIP(hipMemsetAsync(d_long, 0, sizeof(int) * size_t(nbr), s));
+    hipLaunchKernelGGL(k_sort_rows, dim3(nbr), dim3(256), 0, s, d_row_ptr, d_col_ind, d_long);
+    VBH_CHECK_HIP(hipGetLastError());
+    std::vector<int> lr(nbr);
+    VBH_CHECK_HIP(hipMemcpyAsync(lr.data(), d_long, sizeof(int) * size_t(nbr), hipMemcpyDeviceToHost, s));
+    VBH_CHECK_HIP(hipStreamSynchronize(s));
+    if (std::find(lr.begin(), lr.end(), 1) == lr.end()) return;
+    // rows longer than the LDS sort: on the host
+    const int64_t nnzb = row_ptr[nbr];
+    std::vector<int> ci(static_cast<size_t>(nnzb));
+    VBH_CHECK_HIP(hipMemcpyAsync(ci.data(), d_col_ind, sizeof(int) * size_t(nnzb), hipMemcpyDeviceToHost, s));
+    VBH_CHECK_HIP(hipStreamSynchronize(s));
+    rows_parallel(nbr, row_ptr, [&](int r0, int r1) {
+        for (int i = r0; i < r1; ++i)
+            if (lr[i]) std::sort(ci.begin() + row_ptr[i], ci.begin() + row_ptr[i + 1]);
+    });
+    VBH_CHECK_HIP(hipMemcpyAsync(d_col_ind, ci.data(), sizeof(int) * size_t(nnzb), hipMemcpyHostToDevice, s));
+    VBH_CHECK_HIP(hipStreamSynchronize(s));
 }
 
 void destroy_matrix(vbh_matrix* m) {
@@ -540,8 +892,9 @@ void ensure_brw(vbh_matrix* m, hipStream_t s) {
     const unsigned grid = unsigned((m->nbr + 3) / 4);
     if (!m->ecol) {
         std::vector<int64_t> eo(size_t(m->nbr) + 1, 0);
+        const std::vector<int64_t>& hv = host_voff(m, s);
         for (int i = 0; i < m->nbr; ++i) {
-            const int64_t nv = m->h_voff[m->h_row_ptr[i + 1]] - m->h_voff[m->h_row_ptr[i]];
+            const int64_t nv = hv[m->h_row_ptr[i + 1]] - hv[m->h_row_ptr[i]];
             eo[i + 1] = eo[i] + (m->h_rbs[i] > 0 ? nv / m->h_rbs[i] : 0);
         }
         int64_t* d_eo = upload(eo, s);
@@ -593,12 +946,13 @@ void ensure_adjoint_plan(vbh_matrix* m, hipStream_t s) {
     std::lock_guard<std::mutex> lk(m->lazy_mu);
     if (m->t_valid) return;
     std::vector<int> tp(m->nbc + 1, 0), tr(m->nnzb), ts(m->nnzb);
-    for (int64_t k = 0; k < m->nnzb; ++k) tp[m->h_col_ind[k] + 1]++;
+    const std::vector<int>& hc = host_col_ind(m, s);
+    for (int64_t k = 0; k < m->nnzb; ++k) tp[hc[k] + 1]++;
     for (int j = 0; j < m->nbc; ++j) tp[j + 1] += tp[j];
     std::vector<int> fill(tp.begin(), tp.end() - 1);
     for (int i = 0; i < m->nbr; ++i)
         for (int k = m->h_row_ptr[i]; k < m->h_row_ptr[i + 1]; ++k) {
-            const int j = m->h_col_ind[k];
+            const int j = hc[k];
             tr[fill[j]] = i;
             ts[fill[j]] = k;
             ++fill[j];
@@ -628,14 +982,35 @@ void ensure_adjoint_plan(vbh_matrix* m, hipStream_t s) {
     m->t_valid = true;
 }
 
-static std::vector<int> diag_slots(const vbh_matrix* m, const int* diag_col) {
-    std::vector<int> d(m->nbr, -1);
-    for (int i = 0; i < m->nbr; ++i) {
-        const int want = diag_col ? diag_col[i] : i;
-        if (want < 0) continue;
-        for (int k = m->h_row_ptr[i]; k < m->h_row_ptr[i + 1]; ++k)
-            if (m->h_col_ind[k] == want) { d[i] = k; break; }
+// Slot of the block (i, want[i]) of every row (want = i when null; -1 when
+// absent or want[i] < 0).  One wavefront per row.
+__global__ __launch_bounds__(256) void k_diag_slots(const int* __restrict__ row_ptr, const int* __restrict__ col_ind,
+                                                    int nbr, const int* __restrict__ want, int* __restrict__ out) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= nbr) return;
+    const int w = want ? want[i] : i;
+    int found = -1;
+    if (w >= 0)
+        for (int k = row_ptr[i] + lane; k < row_ptr[i + 1]; k += 64)
+            if (col_ind[k] == w) found = k;
+    for (int d = 32; d >= 1; d >>= 1) found = max(found, __shfl_xor(found, d, 64));
+    if (lane == 0) out[i] = found;
+}
+
+// Device array of the diagonal slots (k_diag_slots), owned by tmp.
+static int* diag_slots(const vbh_matrix* m, const int* diag_col_host, DevList& tmp, hipStream_t s) {
+    int* d = tmp.own((int*)dalloc(sizeof(int) * size_t(std::max(m->nbr, 1))));
+    const int* want = nullptr;
+    std::vector<int> h;
+    if (diag_col_host) {
+        h.assign(diag_col_host, diag_col_host + m->nbr);
+        want = tmp.own(upload(h, s));
     }
+    hipLaunchKernelGGL(k_diag_slots, dim3(blocks_for(m->nbr, 4)), dim3(256), 0, s, m->row_ptr, m->col_ind, m->nbr,
+                       want, d);
+    VBH_CHECK_HIP(hipGetLastError());
+    VBH_CHECK_HIP(hipStreamSynchronize(s));  // the upload reads h
     return d;
 }
 
@@ -684,9 +1059,8 @@ __global__ __launch_bounds__(256) void k_keep_compact(const int* __restrict__ ro
 }
 
 // Filtered copy of m (blocks whose norm is below thr dropped), or null when
-// every block is kept.  The keep test and the compaction run on the device;
-// the host receives the per-row counts and the kept columns, which the
-// matrix handle mirrors.
+// every block is kept.  The keep test, the compaction and the new pattern's
+// layout run on the device; the host receives the per-row counts.
 vbh_matrix* filter_by_norms(vbh_matrix* m, double thr, const double* d_norm, bool squared, hipStream_t s) {
     if (m->nnzb == 0) return nullptr;
     DevList tmp;
@@ -701,18 +1075,15 @@ vbh_matrix* filter_by_norms(vbh_matrix* m, double thr, const double* d_norm, boo
     for (int i = 0; i < m->nbr; ++i) rp[i + 1] += rp[i];
     const int64_t total = rp[m->nbr];
     if (total == m->nnzb) return nullptr;
-    std::vector<int> ci(total);
     int* d_src = tmp.own((int*)dalloc(sizeof(int) * size_t(std::max<int64_t>(total, 1))));
+    int* d_ci = tmp.own((int*)dalloc(sizeof(int) * size_t(std::max<int64_t>(total, 1))));
     if (total > 0) {
         int* d_rp = tmp.own(upload(rp, s));
-        int* d_ci = tmp.own((int*)dalloc(sizeof(int) * size_t(total)));
         hipLaunchKernelGGL(k_keep_compact, dim3(nb), dim3(256), 0, s, m->row_ptr, m->col_ind, m->nbr, d_norm,
                            squared ? 1 : 0, thr, d_rp, d_ci, d_src);
         VBH_CHECK_HIP(hipGetLastError());
-        VBH_CHECK_HIP(hipMemcpyAsync(ci.data(), d_ci, sizeof(int) * size_t(total), hipMemcpyDeviceToHost, s));
-        VBH_CHECK_HIP(hipStreamSynchronize(s));
     }
-    MatrixGuard g(build_matrix(m->nbr, m->nbc, m->h_rbs.data(), m->h_cbs.data(), rp.data(), ci.data(), m->dtype, s));
+    MatrixGuard g(build_matrix_dev(m->nbr, m->nbc, m->h_rbs.data(), m->h_cbs.data(), rp.data(), d_ci, m->dtype, s));
     vbh_matrix* c = g.m;
     if (c->nnzb) {
         dispatch_dtype(m->dtype, [&](auto tag) {
@@ -844,7 +1215,17 @@ VBH_API int vbh_matrix_get_pattern(vbh_matrix_t m, int* row_ptr, int* col_ind) {
     return guard([&] {
         VBH_REQUIRE(m, "null matrix");
         std::copy(m->h_row_ptr.begin(), m->h_row_ptr.end(), row_ptr);
-        std::copy(m->h_col_ind.begin(), m->h_col_ind.end(), col_ind);
+        {
+            std::lock_guard<std::mutex> lk(m->host_mu);
+            if (m->h_cols_valid) {
+                std::copy(m->h_col_ind.begin(), m->h_col_ind.end(), col_ind);
+                return;
+            }
+        }
+        // a device-made pattern: straight from the device (the pattern is
+        // complete there once the handle exists), without keeping a mirror
+        if (m->nnzb > 0)
+            VBH_CHECK_HIP(hipMemcpy(col_ind, m->col_ind, sizeof(int) * size_t(m->nnzb), hipMemcpyDeviceToHost));
     });
 }
 
@@ -895,8 +1276,13 @@ static vbh_matrix* clone_matrix(const vbh_matrix* m, hipStream_t s) {
         c->h_rbs = m->h_rbs;
         c->h_cbs = m->h_cbs;
         c->h_row_ptr = m->h_row_ptr;
-        c->h_col_ind = m->h_col_ind;
-        c->h_voff = m->h_voff;
+        {
+            std::lock_guard<std::mutex> lk(m->host_mu);
+            c->h_cols_valid = m->h_cols_valid;
+            c->h_voff_valid = m->h_voff_valid;
+            if (m->h_cols_valid) c->h_col_ind = m->h_col_ind;
+            if (m->h_voff_valid) c->h_voff = m->h_voff;
+        }
         c->h_roff = m->h_roff;
         c->h_coff = m->h_coff;
         c->h_bfs = m->h_bfs;
@@ -936,12 +1322,13 @@ VBH_API int vbh_matrix_transpose(vbh_matrix_t m, int conj, vbh_matrix_t* out, vo
         // within each column), source slots and their block columns
         const int nbr = m->nbr, nbc = m->nbc;
         std::vector<int> tp(nbc + 1, 0), tr(m->nnzb), src(m->nnzb), scol(m->nnzb);
-        for (int64_t k = 0; k < m->nnzb; ++k) tp[m->h_col_ind[k] + 1]++;
+        const std::vector<int>& hc = host_col_ind(m, s);
+        for (int64_t k = 0; k < m->nnzb; ++k) tp[hc[k] + 1]++;
         for (int j = 0; j < nbc; ++j) tp[j + 1] += tp[j];
         std::vector<int> fill(tp.begin(), tp.end() - 1);
         for (int i = 0; i < nbr; ++i)
             for (int k = m->h_row_ptr[i]; k < m->h_row_ptr[i + 1]; ++k) {
-                const int j = m->h_col_ind[k];
+                const int j = hc[k];
                 const int q = fill[j]++;
                 tr[q] = i;
                 src[q] = k;
@@ -997,9 +1384,8 @@ VBH_API int vbh_matrix_shift(vbh_matrix_t m, const double* alpha, const int* dia
         VBH_REQUIRE(m && alpha, "null argument");
         hipStream_t s = (hipStream_t)stream;
         if (m->nbr == 0) return;
-        auto ds = diag_slots(m, diag_col_host);
         DevList tmp;
-        int* d = tmp.own(upload(ds, s));
+        int* d = diag_slots(m, diag_col_host, tmp, s);
         dispatch_dtype(m->dtype, [&](auto tag) {
             using T = decltype(tag);
             hipLaunchKernelGGL(k_shift<T>, dim3(blocks_for(m->nbr, 256)), dim3(256), 0, s, (T*)m->vals, m->voff, d,
@@ -1050,8 +1436,7 @@ VBH_API int vbh_matrix_axpby(vbh_matrix_t self, const double* alpha, vbh_matrix_
         VBH_REQUIRE(self->nbr == X->nbr, "Matrix row count mismatch in axpby");
         *out = nullptr;
         hipStream_t s = (hipStream_t)stream;
-        const bool same = self->h_row_ptr == X->h_row_ptr && self->h_col_ind == X->h_col_ind &&
-                          self->h_rbs == X->h_rbs && self->h_cbs == X->h_cbs;
+        const bool same = same_pattern(self, X, s);
         if (same) {
             if (self->nnz == 0) return;
             dispatch_dtype(self->dtype, [&](auto tag) {
@@ -1078,47 +1463,38 @@ VBH_API int vbh_matrix_axpby(vbh_matrix_t self, const double* alpha, vbh_matrix_
             VBH_REQUIRE(a == 0 || b == 0 || a == b, "axpby: column block sizes differ");
             cbs[j] = a ? a : b;
         }
-        // two passes over row ranges in parallel: merged lengths, then the
-        // merged columns with the source slot of each side (-1: absent)
-        std::vector<int> rp(self->nbr + 1, 0), ci, ys, xs;
-        auto merge_row = [&](int i, int* out_c, int* out_y, int* out_x) {
-            int ky = self->h_row_ptr[i], kx = X->h_row_ptr[i], n = 0;
-            const int ey = self->h_row_ptr[i + 1], ex = X->h_row_ptr[i + 1];
-            while (ky < ey || kx < ex) {
-                const int cy = ky < ey ? self->h_col_ind[ky] : INT32_MAX;
-                const int cx = kx < ex ? X->h_col_ind[kx] : INT32_MAX;
-                const int col = std::min(cy, cx);
-                if (out_c) {
-                    out_c[n] = col;
-                    out_y[n] = cy == col ? ky : -1;
-                    out_x[n] = cx == col ? kx : -1;
-                }
-                if (cy == col) ++ky;
-                if (cx == col) ++kx;
-                ++n;
-            }
-            return n;
-        };
-        rows_parallel(self->nbr, self->h_row_ptr.data(), [&](int r0, int r1) {
-            for (int i = r0; i < r1; ++i) rp[i + 1] = merge_row(i, nullptr, nullptr, nullptr);
-        });
-        for (int i = 0; i < self->nbr; ++i) {
+        // on the device, two passes: merged lengths, then the merged columns
+        // with the source slot of each side (-1: absent)
+        const int nbr = self->nbr;
+        std::vector<int> rp(nbr + 1, 0);
+        DevList ut;
+        int* d_cnt = ut.own((int*)dalloc(sizeof(int) * size_t(std::max(nbr, 1))));
+        if (nbr > 0) {
+            hipLaunchKernelGGL(k_union_rows, dim3(blocks_for(nbr, 4)), dim3(256), 0, s, nbr, self->row_ptr,
+                               self->col_ind, X->row_ptr, X->col_ind, (const int*)nullptr, d_cnt, (int*)nullptr,
+                               (int*)nullptr, (int*)nullptr);
+            VBH_CHECK_HIP(hipGetLastError());
+            VBH_CHECK_HIP(hipMemcpyAsync(rp.data() + 1, d_cnt, sizeof(int) * size_t(nbr), hipMemcpyDeviceToHost, s));
+            VBH_CHECK_HIP(hipStreamSynchronize(s));
+        }
+        for (int i = 0; i < nbr; ++i) {
             VBH_REQUIRE(int64_t(rp[i]) + rp[i + 1] <= int64_t(INT32_MAX), "axpby: union exceeds 2^31 blocks");
             rp[i + 1] += rp[i];
         }
-        ci.resize(size_t(rp[self->nbr]));
-        ys.resize(ci.size());
-        xs.resize(ci.size());
-        rows_parallel(self->nbr, rp.data(), [&](int r0, int r1) {
-            for (int i = r0; i < r1; ++i) merge_row(i, ci.data() + rp[i], ys.data() + rp[i], xs.data() + rp[i]);
-        });
-        MatrixGuard g(build_matrix(self->nbr, nbc, self->h_rbs.data(), cbs.data(), rp.data(), ci.data(),
-                                   self->dtype, s));
+        const int64_t total = rp[nbr];
+        int* d_ci = ut.own((int*)dalloc(sizeof(int) * size_t(std::max<int64_t>(total, 1))));
+        int* d_ys = ut.own((int*)dalloc(sizeof(int) * size_t(std::max<int64_t>(total, 1))));
+        int* d_xs = ut.own((int*)dalloc(sizeof(int) * size_t(std::max<int64_t>(total, 1))));
+        if (total > 0) {
+            int* d_rp = ut.own(upload(rp, s));
+            hipLaunchKernelGGL(k_union_rows, dim3(blocks_for(nbr, 4)), dim3(256), 0, s, nbr, self->row_ptr,
+                               self->col_ind, X->row_ptr, X->col_ind, (const int*)d_rp, (int*)nullptr, d_ci, d_ys,
+                               d_xs);
+            VBH_CHECK_HIP(hipGetLastError());
+        }
+        MatrixGuard g(build_matrix_dev(nbr, nbc, self->h_rbs.data(), cbs.data(), rp.data(), d_ci, self->dtype, s));
         vbh_matrix* c = g.m;
         if (c->nnzb) {
-            DevList tmp;
-            int* d_ys = tmp.own(upload(ys, s));
-            int* d_xs = tmp.own(upload(xs, s));
             dispatch_dtype(self->dtype, [&](auto tag) {
                 using T = decltype(tag);
                 hipLaunchKernelGGL(k_union_axpby<T>, dim3(blocks_for(c->nnzb, 4)), dim3(256), 0, s,
@@ -1138,9 +1514,8 @@ VBH_API int vbh_matrix_trace(vbh_matrix_t m, double* out, void* stream) {
         hipStream_t s = (hipStream_t)stream;
         out[0] = 0.0;
         if (m->nbr == 0) return;
-        auto ds = diag_slots(m, nullptr);
         DevList tmp;
-        int* d = tmp.own(upload(ds, s));
+        int* d = diag_slots(m, nullptr, tmp, s);
         const int nb = 64;
         double* part = (double*)scratch(sizeof(double) * nb, 1);
         dispatch_dtype(m->dtype, [&](auto tag) {

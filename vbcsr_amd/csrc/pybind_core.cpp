@@ -378,14 +378,18 @@ struct Mat {
     vbh_matrix_t h = nullptr;  // owns the device values
     Fresh fresh = Fresh::Both;
     std::vector<T> host;
-    std::vector<int64_t> voff;
+    // value offset of every slot (and the total), computed on first host use:
+    // products and sums that stay on the device never build it
+    std::vector<int64_t> voff_;
+    bool voff_ok = false;
+    int64_t nnz_ = 0;
     uint32_t page_size_ = 0;
 
     explicit Mat(GraphP g) : graph(std::move(g)) { create_from_graph(); }
     // The host mirror of a device-made matrix is allocated on first use
     // (to_host): products and sums of large matrices never touch it.
     Mat(GraphP g, vbh_matrix_t handle) : graph(std::move(g)), h(handle) {
-        compute_voff();
+        nnz_ = info().nnz;
         fresh = Fresh::Device;
     }
     ~Mat() {
@@ -408,6 +412,7 @@ struct Mat {
         const std::vector<int>& rp = graph->adj_ptr;
         const std::vector<int>& ci = graph->adj_ind;
         const std::vector<int>& bs = graph->block_sizes;
+        std::vector<int64_t>& voff = voff_;
         voff.assign(ci.size() + 1, 0);
         // per-row value counts, their prefix, then every slot's offset; the
         // row passes run on up to 16 host threads for large patterns
@@ -443,8 +448,14 @@ struct Mat {
                 }
             }
         });
+        nnz_ = voff.back();
+        voff_ok = true;
     }
-    int64_t nnz() const { return voff.back(); }
+    const std::vector<int64_t>& voff() {
+        if (!voff_ok) compute_voff();
+        return voff_;
+    }
+    int64_t nnz() const { return nnz_; }
     int64_t nnzb() const { return int64_t(graph->adj_ind.size()); }
     vbh_matrix_t dev() {
         if (fresh == Fresh::Host) {
@@ -508,7 +519,7 @@ struct Mat {
                                      "), col " + std::to_string(lcol) + " expects " + std::to_string(c) + " (got " +
                                      std::to_string(data.shape(1)) + ")");
         to_host();
-        T* dst = host.data() + voff[slot];
+        T* dst = host.data() + voff()[slot];
         const T* src = data.data();
         if (mode == 0) std::memcpy(dst, src, sizeof(T) * r * c);  // INSERT
         else
@@ -585,7 +596,7 @@ struct Mat {
             const int slot = find_slot(i, i);
             if (slot < 0) continue;
             const int r = graph->block_sizes[i];
-            for (int t = 0; t < r; ++t) host[voff[slot] + int64_t(t) * r + t] += v.buf.host[graph->block_offsets[i] + t];
+            for (int t = 0; t < r; ++t) host[voff()[slot] + int64_t(t) * r + t] += v.buf.host[graph->block_offsets[i] + t];
         }
         host_written();
     }
@@ -612,30 +623,15 @@ struct Mat {
     }
     // Graph of a new pattern over the same owned rows (and the same column
     // space): used by spmm / filter / union axpby results.
-    GraphP graph_for(vbh_matrix_t m) const {
-        vbh_matrix_info i{};
-        check(vbh_matrix_info_get(m, &i), "info");
-        // this matrix's distribution (owned and ghost blocks, sizes) with the
-        // new matrix's adjacency; the old adjacency is not copied
-        auto g = std::make_shared<Graph>();
-        g->rank = graph->rank;
-        g->size = graph->size;
-        g->owned_global_indices = graph->owned_global_indices;
-        g->ghost_global_indices = graph->ghost_global_indices;
-        g->block_sizes = graph->block_sizes;
-        g->g2l = graph->g2l;
-        g->adj_ptr.assign(i.nbr + 1, 0);
-        g->adj_ind.assign(i.nnzb, 0);
-        check(vbh_matrix_get_pattern(m, g->adj_ptr.data(), g->adj_ind.data()), "pattern");
-        g->finish();
-        return g;
-    }
+    GraphP graph_for(vbh_matrix_t m) const { return graph_for_pattern(m, *graph); }
     void adopt(vbh_matrix_t m) {
         GraphP g = graph_for(m);
         if (h) vbh_matrix_destroy(h);
         h = m;
         graph = g;
-        compute_voff();
+        voff_ok = false;
+        std::vector<int64_t>().swap(voff_);
+        nnz_ = info().nnz;
         std::vector<T>().swap(host);
         fresh = Fresh::Device;
     }
@@ -700,6 +696,7 @@ struct Mat {
         std::ofstream file(filename);
         if (!file.is_open()) throw std::runtime_error("Could not open file for writing: " + filename);
         to_host();
+        const std::vector<int64_t>& vo = voff();
         constexpr bool cx = std::is_same_v<T, cplx>;
         file << "%%MatrixMarket matrix coordinate " << (cx ? "complex" : "real") << " general\n";
         const int n = graph->n_owned();
@@ -718,7 +715,7 @@ struct Mat {
                 const int col1 = graph->block_offsets[j] + 1;
                 for (int a = 0; a < r; ++a)
                     for (int b = 0; b < c; ++b) {
-                        const T v = host[voff[k] + int64_t(a) * c + b];
+                        const T v = host[vo[k] + int64_t(a) * c + b];
                         file << (row1 + a) << " " << (col1 + b) << " ";
                         if constexpr (cx) file << v.real() << " " << v.imag();
                         else file << v;
@@ -764,7 +761,14 @@ struct Mat {
     static GraphP graph_for_pattern(vbh_matrix_t m, const Graph& rows_of) {
         vbh_matrix_info i{};
         check(vbh_matrix_info_get(m, &i), "info");
-        auto g = std::make_shared<Graph>(rows_of);
+        // rows_of's distribution with m's adjacency (rows_of's is not copied)
+        auto g = std::make_shared<Graph>();
+        g->rank = rows_of.rank;
+        g->size = rows_of.size;
+        g->owned_global_indices = rows_of.owned_global_indices;
+        g->ghost_global_indices = rows_of.ghost_global_indices;
+        g->block_sizes = rows_of.block_sizes;
+        g->g2l = rows_of.g2l;
         g->adj_ptr.assign(i.nbr + 1, 0);
         g->adj_ind.assign(i.nnzb, 0);
         check(vbh_matrix_get_pattern(m, g->adj_ptr.data(), g->adj_ind.data()), "pattern");
@@ -796,11 +800,12 @@ struct Mat {
         to_host();
         const int r = graph->block_sizes[lrow], c = graph->block_sizes[lcol];
         py::array_t<T> out({r, c});
-        std::memcpy(out.mutable_data(), host.data() + voff[slot], sizeof(T) * r * c);
+        std::memcpy(out.mutable_data(), host.data() + voff()[slot], sizeof(T) * r * c);
         return out;
     }
     py::array_t<T> to_dense() {
         to_host();
+        const std::vector<int64_t>& vo = voff();
         const int n = graph->n_owned();
         const int rows = graph->owned_scalar_rows(), cols = graph->local_scalar_cols();
         py::array_t<T> out({rows, cols});
@@ -813,7 +818,7 @@ struct Mat {
                 for (int a = 0; a < r; ++a)
                     for (int b = 0; b < c; ++b)
                         o[int64_t(graph->block_offsets[i] + a) * cols + graph->block_offsets[j] + b] =
-                            host[voff[k] + int64_t(a) * c + b];
+                            host[vo[k] + int64_t(a) * c + b];
             }
         return out;
     }
@@ -823,6 +828,7 @@ struct Mat {
             throw std::runtime_error("from_dense: expected shape (" + std::to_string(rows) + ", " +
                                      std::to_string(cols) + ")");
         to_host();
+        const std::vector<int64_t>& vo = voff();
         const T* src = a.data();
         for (int i = 0; i < graph->n_owned(); ++i)
             for (int k = graph->adj_ptr[i]; k < graph->adj_ptr[i + 1]; ++k) {
@@ -830,7 +836,7 @@ struct Mat {
                 const int r = graph->block_sizes[i], c = graph->block_sizes[j];
                 for (int aa = 0; aa < r; ++aa)
                     for (int b = 0; b < c; ++b)
-                        host[voff[k] + int64_t(aa) * c + b] =
+                        host[vo[k] + int64_t(aa) * c + b] =
                             src[int64_t(graph->block_offsets[i] + aa) * cols + graph->block_offsets[j] + b];
             }
         host_written();

@@ -1201,6 +1201,37 @@ __global__ void k_fill_f64(double* __restrict__ p, int64_t n, double v) {
 
 static inline unsigned nblk4(int64_t n) { return unsigned((n + 3) / 4); }
 
+// Panel offsets of a pattern's slots (cs[k] = scalar columns of the row's
+// blocks before slot k, from meta[k].y >> 8) and each row's panel width
+// (w[i]).  One wavefront per row.
+__global__ __launch_bounds__(256) void k_panel_cols(int nbr, const int* __restrict__ row_ptr,
+                                                    const int2* __restrict__ meta, int* __restrict__ cs,
+                                                    int* __restrict__ w) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= nbr) return;
+    const int kb = row_ptr[i], ke = row_ptr[i + 1];
+    for (int k = kb + lane; k < ke; k += 64) cs[k] = meta[k].y >> 8;
+    if (lane == 0) w[i] = ke > kb ? (meta[ke - 1].y >> 8) + (meta[ke - 1].y & 255) : 0;
+}
+
+// Candidate bound of every row of A B: the sum of the lengths of the B rows
+// that row i of A reaches.  One wavefront per row.
+__global__ __launch_bounds__(256) void k_row_bounds(int nbr, const int* __restrict__ a_row_ptr,
+                                                    const int* __restrict__ a_col_ind,
+                                                    const int* __restrict__ b_row_ptr, int64_t* __restrict__ bound) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= nbr) return;
+    int64_t b = 0;
+    for (int ka = a_row_ptr[i] + lane; ka < a_row_ptr[i + 1]; ka += 64) {
+        const int k = a_col_ind[ka];
+        b += b_row_ptr[k + 1] - b_row_ptr[k];
+    }
+    for (int d = 32; d >= 1; d >>= 1) b += __shfl_xor(b, d, 64);
+    if (lane == 0) bound[i] = b;
+}
+
 }  // namespace vbh
 
 using namespace vbh;
@@ -1239,14 +1270,18 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
         // ---- symbolic sizing (host)
         std::vector<int64_t> bound(nbr, 0), out_off(nbr + 1, 0);
         std::vector<int> tsize(nbr, 1);
+        if (nbr > 0) {
+            int64_t* d_bound = tmp.own((int64_t*)dalloc(sizeof(int64_t) * size_t(nbr)));
+            hipLaunchKernelGGL(k_row_bounds, dim3(nblk4(nbr)), dim3(256), 0, s, nbr, A->row_ptr, A->col_ind,
+                               B->row_ptr, d_bound);
+            VBH_CHECK_HIP(hipGetLastError());
+            VBH_CHECK_HIP(hipMemcpyAsync(bound.data(), d_bound, sizeof(int64_t) * size_t(nbr), hipMemcpyDeviceToHost,
+                                         s));
+            VBH_CHECK_HIP(hipStreamSynchronize(s));
+        }
         rows_parallel(nbr, A->h_row_ptr.data(), [&](int r0, int r1) {
             for (int i = r0; i < r1; ++i) {
-                int64_t b = 0;
-                for (int ka = A->h_row_ptr[i]; ka < A->h_row_ptr[i + 1]; ++ka) {
-                    const int k = A->h_col_ind[ka];
-                    b += B->h_row_ptr[k + 1] - B->h_row_ptr[k];
-                }
-                bound[i] = b;
+                const int64_t b = bound[i];
                 const int64_t cap = std::min<int64_t>(b, B->nbc);
                 out_off[i + 1] = cap;
                 int64_t S = 1;
@@ -1386,22 +1421,16 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
             crp[i + 1] = crp[i] + counts[i];
         }
         const int64_t total = crp[nbr];
-        std::vector<int> cci(total);
+        DevList ct;
+        int* d_cci = ct.own((int*)dalloc(sizeof(int) * size_t(std::max<int64_t>(total, 1))));
         if (total > 0) {
-            DevList ct;
             int* d_crp = ct.own(upload(crp, s));
-            int* d_cci = ct.own((int*)dalloc(sizeof(int) * size_t(total)));
             hipLaunchKernelGGL(k_compact_cols, dim3(nbr), dim3(256), 0, s, d_out_cols, d_out_off, d_crp, d_cci);
             VBH_CHECK_HIP(hipGetLastError());
-            VBH_CHECK_HIP(hipMemcpyAsync(cci.data(), d_cci, sizeof(int) * size_t(total), hipMemcpyDeviceToHost, s));
-            VBH_CHECK_HIP(hipStreamSynchronize(s));
+            sort_rows_dev(nbr, crp.data(), d_crp, d_cci, s);
         }
-        rows_parallel(nbr, crp.data(), [&](int r0, int r1) {
-            for (int i = r0; i < r1; ++i)
-                if (!std::is_sorted(cci.begin() + crp[i], cci.begin() + crp[i + 1]))
-                    std::sort(cci.begin() + crp[i], cci.begin() + crp[i + 1]);
-        });
-        MatrixGuard C(build_matrix(nbr, B->nbc, A->h_rbs.data(), B->h_cbs.data(), crp.data(), cci.data(), A->dtype, s));
+        MatrixGuard C(build_matrix_dev(nbr, B->nbc, A->h_rbs.data(), B->h_cbs.data(), crp.data(), d_cci, A->dtype,
+                                       s));
         unsigned long long* d_np = tmp.own((unsigned long long*)dalloc(sizeof(unsigned long long)));
         vbh_matrix* Cm = C.m;
         VBH_CHECK_HIP(hipMemsetAsync(d_np, 0, sizeof(unsigned long long), s));
@@ -1553,17 +1582,13 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
             // matrix cores.
             const bool bf_owned = numeric == 2 && A->max_block <= 8 && B->max_block <= 8;
             if (numeric == 2 && !bf_owned) {
-                std::vector<int> colstart(Cm->nnzb), width(nbr);
-                for (int i = 0; i < nbr; ++i) {
-                    int w = 0;
-                    for (int q = crp[i]; q < crp[i + 1]; ++q) {
-                        colstart[q] = w;
-                        w += B->h_cbs[cci[q]];
-                    }
-                    width[i] = w;
-                }
-                int* d_cs = tmp.own(upload(colstart, s));
-                int* d_w = tmp.own(upload(width, s));
+                // each C slot's first scalar column within its row's panel and
+                // the panel widths, from C's slot metadata
+                int* d_cs = tmp.own((int*)dalloc(sizeof(int) * size_t(Cm->nnzb)));
+                int* d_w = tmp.own((int*)dalloc(sizeof(int) * size_t(std::max(nbr, 1))));
+                hipLaunchKernelGGL(k_panel_cols, dim3(nblk4(nbr)), dim3(256), 0, s, nbr, Cm->row_ptr, Cm->meta, d_cs,
+                                   d_w);
+                VBH_CHECK_HIP(hipGetLastError());
                 MfmaArgs ma{A->row_ptr, A->col_ind, A->voff, (const double*)A->vals, an, A->rbs, A->cbs,
                             B->row_ptr, B->col_ind, B->voff, (const double*)B->vals, bn, Cm->row_ptr, Cm->col_ind,
                             Cm->voff, Cm->cbs, d_cs, d_w, (double*)Cm->vals, thr, d_cnorm, d_np};
@@ -1693,7 +1718,7 @@ static void complete_hermitian_impl(vbh_matrix_t U, vbh_matrix_t* out, hipStream
     for (int i = 0; i < n; ++i)
         VBH_REQUIRE(U->h_rbs[i] == U->h_cbs[i], "Hermitian completion needs equal row and column block sizes");
     const std::vector<int>& rp = U->h_row_ptr;
-    const std::vector<int>& ci = U->h_col_ind;
+    const std::vector<int>& ci = host_col_ind(U, s);
     // rows of C: mirrored blocks (column j < i, from U's row j) then U's own
     // blocks with j >= i; U blocks below the diagonal are ignored
     std::vector<std::vector<std::pair<int, int>>> lower(n);  // (col, U slot)
