@@ -495,6 +495,27 @@ def main():
                                        f"{B.nnz * 8 / 1e9:.2f} GB FP64 values",
                              "ms": ms, "gbs": gbs, "frac_hbm": gbs / HBM_PEAK_GBS, "alg_bytes": bts}
         log(f"spmv large: {ms:.3f} ms, {gbs:.1f} GB/s ({100 * gbs / HBM_PEAK_GBS:.1f}% of HBM peak)")
+        # the box's streaming ceiling on the same byte count: a plain read
+        # (torch sum over FP64) and a copy (half read, half written)
+        buf = torch.empty(bts // 8, dtype=torch.float64, device="cuda").uniform_()
+        dst = torch.empty(bts // 16, dtype=torch.float64, device="cuda")
+        half = buf[: bts // 16]
+        ceil = {}
+        for name, fn in (("read", lambda: buf.sum()), ("copy", lambda: dst.copy_(half))):
+            fn()
+            sync()
+            ev0.record()
+            for _ in range(reps):
+                fn()
+            ev1.record()
+            sync()
+            t = ev0.elapsed_time(ev1) / reps * 1e-3
+            ceil[f"{name}_gbs"] = (8 * (bts // 8) if name == "read" else 16 * (bts // 16)) / t / 1e9
+        del buf, dst, half
+        ceil["spmv_frac_of_read"] = gbs / ceil["read_gbs"]
+        out["spmv_large"]["stream_ceiling"] = ceil
+        log(f"streaming ceiling on {bts / 1e9:.2f} GB: read {ceil['read_gbs']:.0f} GB/s, copy {ceil['copy_gbs']:.0f} "
+            f"GB/s; SpMV at {100 * ceil['spmv_frac_of_read']:.0f}% of the read rate")
         if not args.no_cpu and not args.quick:
             import oracle
 
