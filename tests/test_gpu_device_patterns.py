@@ -1,0 +1,131 @@
+"""Patterns made on the device (SpGEMM results, union sums, filtered copies)
+against the oracle: the row sort of the SpGEMM result (LDS network and the
+host path for rows longer than 4096 columns), the device union merge with
+empty rows on either side, a filter that drops every block, and the host
+paths that read a device-made pattern on first use (transpose, adjoint apply,
+shift and trace, blocks, dense export).  Patterns identical, FP64 values
+1e-12 relative."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def mk(A):
+    from vbcsr_amd import VBCSR
+
+    return VBCSR.from_block_csr(A.bs, A.row_ptr, A.col_ind, A.vals)
+
+
+def with_empty_rows(A, rows, seed):
+    """A copy of A with the given block rows emptied (values renumbered)."""
+    from vbcsr_amd.workloads import BlockCSR
+
+    keep = np.ones(A.nnzb, bool)
+    for i in rows:
+        keep[A.row_ptr[i]:A.row_ptr[i + 1]] = False
+    cnt = np.diff(A.row_ptr).copy()
+    cnt[list(rows)] = 0
+    rp = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int32)
+    ci = A.col_ind[keep].astype(np.int32)
+    B = BlockCSR(A.bs, rp, ci, np.zeros(0))
+    B.vals = np.random.default_rng(seed).uniform(-1, 1, int(B.value_offsets()[-1]))
+    return B
+
+
+def test_long_unsorted_rows():
+    """Global hash tables (thr < 0) with rows of more than 4096 kept columns:
+    the device sort hands those rows to the host, the rest stay on the
+    device.  6000 block rows of size 1, 100 blocks per row: about 4900
+    distinct columns per product row."""
+    from vbcsr_amd import workloads as W
+
+    A = W.random_vbcsr(6000, bs_lo=1, bs_hi=1, per_row=100, seed=21)
+    H = mk(A)
+    C = H.spmm(H, -1.0)
+    Cr = O.spgemm(O.Mat.of(A), O.Mat.of(A), -1.0)
+    assert np.diff(Cr.row_ptr).max() > 4096
+    assert np.array_equal(C.row_ptr, Cr.row_ptr) and np.array_equal(C.col_ind, Cr.col_ind)
+    assert relerr(C.get_values(), Cr.vals) < 1e-12
+
+
+@pytest.mark.parametrize("alpha,beta", [(-2.0, 3.0), (1.0, 0.0), (0.5, -1.0)])
+def test_union_axpby_device_merge(alpha, beta):
+    """Two random patterns over the same rows, each with some rows emptied
+    (empty on one side, on the other, on both), in place (axpby) and as a new
+    matrix (combined: alpha X + beta Y), against the oracle's union."""
+    from vbcsr_amd import workloads as W
+
+    Y0 = W.random_vbcsr(700, bs_lo=1, bs_hi=5, per_row=12, seed=31)
+    X0 = W.random_vbcsr(700, bs_lo=1, bs_hi=5, per_row=12, seed=32)
+    X0.bs = Y0.bs  # same block sizes, different patterns
+    X0 = with_empty_rows(X0, [0, 5, 9, 699], seed=33)
+    Y0 = with_empty_rows(Y0, [5, 7, 9, 698], seed=34)
+    ref = O.axpby(O.Mat.of(Y0), alpha, O.Mat.of(X0), beta)
+    Y, X = mk(Y0), mk(X0)
+    Z = Y.duplicate()
+    Z.axpby(alpha, X, beta)
+    assert np.array_equal(Z.row_ptr, ref.row_ptr) and np.array_equal(Z.col_ind, ref.col_ind)
+    assert relerr(Z.get_values(), ref.vals) < 1e-12
+    S = Y * beta + X * alpha
+    assert np.array_equal(S.row_ptr, ref.row_ptr) and np.array_equal(S.col_ind, ref.col_ind)
+    assert relerr(S.get_values(), ref.vals) < 1e-12
+    # equal patterns (device column comparison) take the flat path
+    Q = Y.duplicate()
+    Q.axpby(alpha, Y.duplicate(), beta)
+    assert np.array_equal(Q.row_ptr, Y.row_ptr) and np.array_equal(Q.col_ind, Y.col_ind)
+    assert relerr(Q.get_values(), (alpha + beta) * np.asarray(Y.get_values())) < 1e-14
+
+
+def test_filter_drops_every_block():
+    from vbcsr_amd import workloads as W
+
+    A = W.random_vbcsr(300, bs_lo=1, bs_hi=4, per_row=6, seed=41)
+    H = mk(A)
+    H.filter_blocks(1e9)
+    assert H.local_block_nnz == 0
+    assert np.array_equal(H.row_ptr, np.zeros(A.nb + 1, np.int32))
+    assert np.asarray(H.get_values()).size == 0
+    # and a product of an empty matrix stays empty
+    E = H.spmm(H, 0.0)
+    assert E.local_block_nnz == 0
+
+
+def test_device_made_matrix_host_paths():
+    """A SpGEMM result used by the paths that need its pattern on the host
+    (adjoint plan, transpose, row classes, blocks, dense export) and by the
+    device diagonal search (shift, trace)."""
+    from vbcsr_amd import workloads as W
+
+    A = W.random_vbcsr(400, bs_lo=1, bs_hi=6, per_row=5, seed=51)
+    H = mk(A)
+    m = O.Mat.of(A)
+    C = H.spmm(H, 0.05)
+    Cr = O.spgemm(m, m, 0.05)
+    assert np.array_equal(C.row_ptr, Cr.row_ptr) and np.array_equal(C.col_ind, Cr.col_ind)
+    D = Cr.dense()
+    assert relerr(C.to_dense(), D) < 1e-12
+    # a block of the result (Python-side value offsets built lazily)
+    i = 17
+    j = int(Cr.col_ind[Cr.row_ptr[i]])
+    ro = np.concatenate([[0], np.cumsum(A.bs)])
+    assert relerr(C.get_block(i, j), D[ro[i]:ro[i + 1], ro[j]:ro[j + 1]]) < 1e-12
+    # SpMV and adjoint SpMV on the result
+    rng = np.random.default_rng(52)
+    x = rng.standard_normal(D.shape[0])
+    xv = C.create_vector()
+    xv.from_numpy(x)
+    y = C.mult(xv)
+    assert relerr(np.asarray(y.to_numpy()), D @ x) < 1e-12
+    ya = C.mult_adjoint(xv)
+    assert relerr(np.asarray(ya.to_numpy()), D.T @ x) < 1e-12
+    # transpose of the device-made pattern
+    T = C.transpose()
+    assert relerr(T.to_dense(), D.T) < 1e-12
+    # trace and shift through the device diagonal search
+    assert abs(C.trace() - np.trace(D)) < 1e-9 * max(1.0, abs(np.trace(D)))
+    C.shift(0.25)
+    assert relerr(C.get_values(), O.shift(Cr, 0.25).vals) < 1e-12
