@@ -1068,49 +1068,83 @@ __global__ __launch_bounds__(256) void k_densify(const T* __restrict__ vals, con
 // ------------------------------------------------------------ BF16 dense products
 // C (FP32, [Mp][Np]) = A (BF16, [Mp][Kp]) times B^T (BF16, [Np][Kp]): both
 // operands k-contiguous, every extent padded with zeros to the tile sizes, so
-// the kernel has no bounds checks.  A work-group computes a 128 x 128 tile of
-// C, each of its 4 waves a 64 x 64 quarter as 4 x 4 tiles of
-// v_mfma_f32_16x16x32_bf16 (FP32 accumulation).  Per k-step of 32 the
-// group stages A's and B's 128 x 32 slices in LDS (row pitch 40 elements so
+// the kernel has no bounds checks.  A work-group computes a kGemmTM x
+// kGemmTN tile of C with one wave per 64 x 64 part, each as 4 x 4 tiles of
+// v_mfma_f32_16x16x32_bf16 (FP32 accumulation).  Per k-step of kGemmK the
+// group stages A's and B's slices in LDS (row pitch kGemmK + 8 elements so
 // the 16 rows an operand read touches fall on different banks); the next
 // step's slices are loaded into registers while the current one is
-// multiplied.
-constexpr int kGemmTile = 128;
-constexpr int kGemmK = 32;
+// multiplied, and one barrier separates the steps.  Measured on a 30k x 30k
+// x 30k product: 128 x 128 tiles with k-steps of 32, 114 ms; 256 x 256, 59
+// ms (half the operand bytes per flop); k-steps of 64, 45 ms (half the
+// barriers); grouped tile order, 43.6 ms (1.26 PFLOP/s).
+#ifndef VBH_GEMM_TM
+#define VBH_GEMM_TM 256
+#endif
+#ifndef VBH_GEMM_TN
+#define VBH_GEMM_TN 256
+#endif
+constexpr int kGemmTM = VBH_GEMM_TM;
+constexpr int kGemmTN = VBH_GEMM_TN;
+#ifndef VBH_GEMM_K
+#define VBH_GEMM_K 64
+#endif
+#ifndef VBH_GEMM_GROUP
+#define VBH_GEMM_GROUP 4
+#endif
+constexpr int kGemmK = VBH_GEMM_K;
 constexpr int kGemmPitch = kGemmK + 8;
+constexpr int kGemmGroup = VBH_GEMM_GROUP;
+constexpr int kGemmWaves = (kGemmTM / 64) * (kGemmTN / 64);
+static_assert(kGemmTM % 64 == 0 && kGemmTN % 64 == 0 && kGemmWaves <= 16, "work-group tile of 64 x 64 wave tiles");
 
-__global__ __launch_bounds__(256) void k_gemm_bf16_nt(const short* __restrict__ A, const short* __restrict__ BT,
-                                                      float* __restrict__ C, int64_t Kp, int64_t ldc, int tiles_n,
-                                                      int n_tiles) {
-    __shared__ __attribute__((aligned(16))) short as[2][kGemmTile * kGemmPitch];
-    __shared__ __attribute__((aligned(16))) short bs[2][kGemmTile * kGemmPitch];
+__global__ __launch_bounds__(64 * kGemmWaves) void k_gemm_bf16_nt(const short* __restrict__ A,
+                                                                  const short* __restrict__ BT,
+                                                                  float* __restrict__ C, int64_t Kp, int64_t ldc,
+                                                                  int tiles_n, int n_tiles) {
+    __shared__ __attribute__((aligned(16))) short as[2][kGemmTM * kGemmPitch];
+    __shared__ __attribute__((aligned(16))) short bs[2][kGemmTN * kGemmPitch];
     // the launch is rounded up to a multiple of 8 for the XCD remapping
     const int gid = xcd_chunk(int(gridDim.x));
     if (gid >= n_tiles) return;
-    const int tm = gid / tiles_n, tn = gid - tm * tiles_n;
+    // consecutive tiles walk kGemmGroup tile rows before the next column, so
+    // the work-groups of one XCD share A's and B's slices in its L2
+    const int tiles_m = n_tiles / tiles_n;
+    const int grp = gid / (kGemmGroup * tiles_n), q = gid - grp * (kGemmGroup * tiles_n);
+    const int gm = tiles_m - grp * kGemmGroup < kGemmGroup ? tiles_m - grp * kGemmGroup : kGemmGroup;
+    const int tm = grp * kGemmGroup + q % gm, tn = q / gm;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
-    const int64_t m0 = int64_t(tm) * kGemmTile, n0 = int64_t(tn) * kGemmTile;
-    // staging: 512 16-byte chunks per operand slice, two per thread
-    // (chunk c: row c / 4, k-offset 8 * (c % 4))
+    const int wm = wave / (kGemmTN / 64), wn = wave % (kGemmTN / 64);
+    const int64_t m0 = int64_t(tm) * kGemmTM, n0 = int64_t(tn) * kGemmTN;
+    // staging: RC 16-byte chunks per operand row and k-step (chunk c: row
+    // c / RC, k-offset 8 * (c % RC)), CA / CB per thread
+    constexpr int NT = 64 * kGemmWaves, RC = kGemmK / 8;
+    constexpr int CA = kGemmTM * RC / NT, CB = kGemmTN * RC / NT;
+    static_assert(CA * NT == kGemmTM * RC && CB * NT == kGemmTN * RC, "whole staging chunks per thread");
     typedef __attribute__((__vector_size__(4 * sizeof(int)))) int i32x4;
-    i32x4 ra[2], rb[2];
+    i32x4 ra[CA], rb[CB];
     auto load = [&](int64_t k0) {
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int c = tid + 256 * u;
-            const int row = c >> 2, kc = (c & 3) * 8;
-            ra[u] = *reinterpret_cast<const i32x4*>(A + (m0 + row) * Kp + k0 + kc);
-            rb[u] = *reinterpret_cast<const i32x4*>(BT + (n0 + row) * Kp + k0 + kc);
+        for (int u = 0; u < CA; ++u) {
+            const int c = tid + NT * u;
+            ra[u] = *reinterpret_cast<const i32x4*>(A + (m0 + c / RC) * Kp + k0 + (c % RC) * 8);
+        }
+#pragma unroll
+        for (int u = 0; u < CB; ++u) {
+            const int c = tid + NT * u;
+            rb[u] = *reinterpret_cast<const i32x4*>(BT + (n0 + c / RC) * Kp + k0 + (c % RC) * 8);
         }
     };
     auto store = [&](int buf) {
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int c = tid + 256 * u;
-            const int row = c >> 2, kc = (c & 3) * 8;
-            *reinterpret_cast<i32x4*>(&as[buf][row * kGemmPitch + kc]) = ra[u];
-            *reinterpret_cast<i32x4*>(&bs[buf][row * kGemmPitch + kc]) = rb[u];
+        for (int u = 0; u < CA; ++u) {
+            const int c = tid + NT * u;
+            *reinterpret_cast<i32x4*>(&as[buf][(c / RC) * kGemmPitch + (c % RC) * 8]) = ra[u];
+        }
+#pragma unroll
+        for (int u = 0; u < CB; ++u) {
+            const int c = tid + NT * u;
+            *reinterpret_cast<i32x4*>(&bs[buf][(c / RC) * kGemmPitch + (c % RC) * 8]) = rb[u];
         }
     };
     f32x4 acc[4][4];
@@ -1126,17 +1160,21 @@ __global__ __launch_bounds__(256) void k_gemm_bf16_nt(const short* __restrict__ 
     for (int st = 0; st < nsteps; ++st) {
         const int buf = st & 1;
         if (st + 1 < nsteps) load(int64_t(st + 1) * kGemmK);
-        bf16x8 av[4], bv[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
-            av[i] = *reinterpret_cast<const bf16x8*>(&as[buf][(wm * 64 + i * 16 + lr) * kGemmPitch + lk]);
+        for (int kk = 0; kk < kGemmK; kk += 32) {
+            bf16x8 av[4], bv[4];
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
-            bv[j] = *reinterpret_cast<const bf16x8*>(&bs[buf][(wn * 64 + j * 16 + lr) * kGemmPitch + lk]);
+            for (int i = 0; i < 4; ++i)
+                av[i] = *reinterpret_cast<const bf16x8*>(&as[buf][(wm * 64 + i * 16 + lr) * kGemmPitch + kk + lk]);
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j)
+                bv[j] = *reinterpret_cast<const bf16x8*>(&bs[buf][(wn * 64 + j * 16 + lr) * kGemmPitch + kk + lk]);
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
+        }
         if (st + 1 < nsteps) store(buf ^ 1);
         __syncthreads();
     }
@@ -1456,11 +1494,11 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
         // pattern blocks and their norms.  Every stored pair is multiplied:
         // the numeric skip rule drops products of norm below thr / nnz_row,
         // under BF16 rounding.
-        const int64_t Mp = (A->n_rows + kGemmTile - 1) / kGemmTile * kGemmTile;
-        const int64_t Np = (B->n_cols + kGemmTile - 1) / kGemmTile * kGemmTile;
+        const int64_t Mp = (A->n_rows + kGemmTM - 1) / kGemmTM * kGemmTM;
+        const int64_t Np = (B->n_cols + kGemmTN - 1) / kGemmTN * kGemmTN;
         const int64_t Kp = (A->n_cols + kGemmK - 1) / kGemmK * kGemmK;
         bool dense16 = numeric == 2 && Cm->nnzb > 0 && A->dtype == VBH_F64 && fill >= kDense16Fill &&
-                       Mp / kGemmTile * (Np / kGemmTile) <= INT32_MAX - 8;
+                       Mp / kGemmTM * (Np / kGemmTN) <= INT32_MAX - 8;
         if (dense16) {
             const size_t need = size_t(2) * size_t(Mp + Np) * size_t(Kp) + size_t(4) * size_t(Mp) * size_t(Np);
             size_t free_b = 0, total_b = 0;
@@ -1484,10 +1522,10 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
                                    (const double*)B->vals, B->voff, B->row_ptr, B->col_ind, B->roff, B->coff, B->cbs,
                                    B->nbr, B->nnzb, Kp, 1, b16);
             VBH_CHECK_HIP(hipGetLastError());
-            const int tiles_n = int(Np / kGemmTile);
-            const int n_tiles = int(Mp / kGemmTile) * tiles_n;
-            hipLaunchKernelGGL(k_gemm_bf16_nt, dim3((n_tiles + 7) & ~7), dim3(256), 0, s, a16, b16, cd, Kp, Np,
-                               tiles_n, n_tiles);
+            const int tiles_n = int(Np / kGemmTN);
+            const int n_tiles = int(Mp / kGemmTM) * tiles_n;
+            hipLaunchKernelGGL(k_gemm_bf16_nt, dim3((n_tiles + 7) & ~7), dim3(64 * kGemmWaves), 0, s, a16, b16, cd,
+                               Kp, Np, tiles_n, n_tiles);
             VBH_CHECK_HIP(hipGetLastError());
             hipLaunchKernelGGL(k_scatter_dense, dim3(nblk4(Cm->nnzb)), dim3(256), 0, s, cd, Np, Cm->row_ptr,
                                Cm->col_ind, Cm->voff, Cm->roff, Cm->coff, Cm->cbs, nbr, Cm->nnzb, (double*)Cm->vals,
