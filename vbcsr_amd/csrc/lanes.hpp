@@ -149,12 +149,18 @@ __device__ __forceinline__ void quad_fma_cols(const A (&ad)[4], const vbh_w4 (&x
 }
 
 // Slot loop of one lane for uniform 4x4 blocks.  The 4 rows of a block row
-// sit in one quad of lanes.  Column indices of 4*CB slots are preloaded
-// lane-parallel (lane a holds slots a, a+4, ...), and U slots are issued per
-// batch, so a row of 10 blocks costs 1 + 10/U memory latencies; U shrinks
-// with the slice width to keep 5+ waves per SIMD resident (measured: U = 2
-// beats 1 and 4 for 10 FP32 columns).  Two ways to read a block's 4 X rows:
-//  * quad-contiguous (qx: one lane slice of all K columns, rows stored
+// sit in one quad of lanes.  Column indices of 4*CB slots are loaded
+// lane-parallel up front (lane a holds slots a, a+4, ...) and complete
+// together, and U slots are issued per batch -- their values, then their X
+// chunks, in one basic block -- so a row of 10 blocks costs about 1 + 10/U
+// memory latencies (measured: U = 2 beats 1 and 4 for 10 FP32 columns).
+// Against a loop that waited for each batch's column index and values before
+// addressing its X rows: the 200 x 200 graphene KPM step 26.4 -> 24.5 us
+// (FP32) and 46.1 -> 38.3 us (FP64); on the 800 x 800 lattice (HBM-resident)
+// FP32 422 -> 431 us and FP64 760 -> 906 us.  Two ways to read a block's 4
+// X rows (QX, a template parameter: a branch on it inside the batch would
+// make the compiler wait for the value loads before the X loads):
+//  * quad-contiguous (QX: one lane slice of all K columns, rows stored
 //    back to back, ldx == K, X 16-byte aligned -- the KPM recursion
 //    buffers): the 4 rows are 4*K contiguous elements, read as 16-byte
 //    chunks, chunk q by lane q % 4 of the quad, so each load instruction
@@ -165,9 +171,9 @@ __device__ __forceinline__ void quad_fma_cols(const A (&ad)[4], const vbh_w4 (&x
 // traffic is not replicated across the block's rows.  Slots past the
 // partition's count are clamped to its last slot and enter with zero A
 // values (one select per A value instead of one per product).
-template <typename T, typename V, int KT, int VN>
+template <typename T, typename V, int KT, int VN, bool QX>
 __device__ __forceinline__ void quad_slots(const ApplyArgs<T, V>& g, int kb, int cnt, int stride, int a, int v0,
-                                           int nvalid, bool qx, typename Wide<T>::type (&acc)[KT]) {
+                                           int nvalid, typename Wide<T>::type (&acc)[KT]) {
     using A = typename Wide<T>::type;
     constexpr int CB = 4;                                   // slots per lane per pass
     constexpr int per = int(KT * sizeof(T) / 4);            // VGPRs of one X slice
@@ -189,10 +195,16 @@ __device__ __forceinline__ void quad_slots(const ApplyArgs<T, V>& g, int kb, int
         int colv[CB];
 #pragma unroll
         for (int t = 0; t < CB; ++t) colv[t] = p.col_ind[k0 + (a + 4 * t < ns ? a + 4 * t : ns - 1) * stride];
+        // the four column loads complete here, together: without this the
+        // compiler sinks each into the batch that first uses it, and every
+        // batch then waits for its column index before it can even address
+        // its X rows (two dependent memory latencies per batch)
+        static_assert(CB == 4, "column preload of four registers");
+        asm volatile("" : "+v"(colv[0]), "+v"(colv[1]), "+v"(colv[2]), "+v"(colv[3]));
 #pragma unroll
         for (int s0 = 0; s0 < 4 * CB; s0 += U) {
             if (s0 >= ns) break;
-            A ad[U][4];
+            V av[U][4];  // raw values, widened where they are used, after the X loads are issued
             int col[U];
 #pragma unroll
             for (int u = 0; u < U; ++u) {
@@ -206,11 +218,20 @@ __device__ __forceinline__ void quad_slots(const ApplyArgs<T, V>& g, int kb, int
                 const int sc = s < ns ? s : ns - 1;
                 const V* ar = static_cast<const V*>(__builtin_assume_aligned(
                     g.vals + int64_t(k0 + sc * stride) * 16 + a * 4, (4 * sizeof(V)) % 16 ? sizeof(V) : 16));
-                const bool on = s < ns;
 #pragma unroll
-                for (int b = 0; b < 4; ++b) ad[u][b] = widen(pick(on, ar[b], Sc<V>::zero()));
+                for (int b = 0; b < 4; ++b) av[u][b] = ar[b];
             }
-            if (qx) {
+            // value loads issue before the X loads (the first products wait
+            // for the oldest loads only)
+            asm volatile("" ::: "memory");
+            auto widened = [&](int u, A (&ad)[4]) {
+                const bool on = s0 + u < ns;
+#pragma unroll
+                for (int b = 0; b < 4; ++b) ad[b] = widen(pick(on, av[u][b], Sc<V>::zero()));
+            };
+            // (the X layout is a template parameter: a branch on it here would
+            // make the compiler wait for the value loads before the X loads)
+            if constexpr (QX) {
                 vbh_w4 xc[U][NCL];
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
@@ -220,7 +241,9 @@ __device__ __forceinline__ void quad_slots(const ApplyArgs<T, V>& g, int kb, int
                 }
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
-                    quad_fma_cols<T, A, KT, NCL, 0>(ad[u], xc[u], acc);
+                    A ad[4];
+                    widened(u, ad);
+                    quad_fma_cols<T, A, KT, NCL, 0>(ad, xc[u], acc);
                 }
             } else {
                 T xr[U][KT];
@@ -229,14 +252,16 @@ __device__ __forceinline__ void quad_slots(const ApplyArgs<T, V>& g, int kb, int
                     load_slice<T, KT, VN>(g.X + (int64_t(col[u]) * 4 + a) * g.ldx + v0, nvalid, xr[u]);
 #pragma unroll
                 for (int u = 0; u < U; ++u) {
+                    A ad[4];
+                    widened(u, ad);
 #pragma unroll
                     for (int j = 0; j < KT; ++j) {
                         const T x0 = quad_bcast<0>(xr[u][j]), x1 = quad_bcast<1>(xr[u][j]);
                         const T x2 = quad_bcast<2>(xr[u][j]), x3 = quad_bcast<3>(xr[u][j]);
-                        A t = Sc<A>::fma(ad[u][0], widen(x0), acc[j]);
-                        t = Sc<A>::fma(ad[u][1], widen(x1), t);
-                        t = Sc<A>::fma(ad[u][2], widen(x2), t);
-                        acc[j] = Sc<A>::fma(ad[u][3], widen(x3), t);
+                        A t = Sc<A>::fma(ad[0], widen(x0), acc[j]);
+                        t = Sc<A>::fma(ad[1], widen(x1), t);
+                        t = Sc<A>::fma(ad[2], widen(x2), t);
+                        acc[j] = Sc<A>::fma(ad[3], widen(x3), t);
                     }
                 }
             }
@@ -295,7 +320,9 @@ __device__ __forceinline__ void lane_rows(const ApplyArgs<T, V>& g, int64_t sb, 
             // quad-contiguous X rows: one slice of all K columns, rows back to
             // back, 16-byte aligned (wave-uniform)
             const bool qx = S == 1 && nvalid == KT && g.ldx == KT && (reinterpret_cast<uintptr_t>(g.X) & 15) == 0;
-            quad_slots<T, V, KT, VN>(g, kp, ke > kp ? (ke - kp + P - 1) / P : 0, P, a, v0, nvalid, qx, acc);
+            const int cnt = ke > kp ? (ke - kp + P - 1) / P : 0;
+            if (qx) quad_slots<T, V, KT, VN, true>(g, kp, cnt, P, a, v0, nvalid, acc);
+            else quad_slots<T, V, KT, VN, false>(g, kp, cnt, P, a, v0, nvalid, acc);
         } else if constexpr (BS > 0) {
             for (int k = kb + part; k < ke; k += P) {
                 const int col = p.col_ind[k];
