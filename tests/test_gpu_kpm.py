@@ -261,3 +261,26 @@ def test_c2_fp32_dos_error_full_lattice():
     err = K.dos_error(K.dos_from_moments(mu32)[1], r64)
     print(f"C2 full lattice FP32 DOS error {err:.2e}")
     assert err <= 1e-5
+
+
+def test_large_fp64_quads_four_slot_batches():
+    """FP64 4x4-block launches with one partition per row (at least 2^19
+    scalar rows) run the quad slot loop with 4-slot batches: graphene 300 x
+    300 cells (N = 720,000), one SpMM with K = 10 against the oracle's SpMM,
+    and 8 fused moments (two per matvec) against the oracle's recursion."""
+    from vbcsr_amd import kpm as K
+    from vbcsr_amd import workloads as W
+
+    A = W.graphene(300, 300, 4, seed=4)
+    assert A.n >= 1 << 19
+    H = mk(A)
+    m = O.Mat.of(A)
+    X = np.random.default_rng(5).standard_normal((A.n, 10))
+    Xv = H.create_multivector(10)
+    Xv.from_numpy(X)
+    Y = H.mult(Xv)
+    assert relerr(np.asarray(Y.to_numpy()), O.spmm(m, X)) < 1e-12
+    V0 = H.create_multivector(10)
+    V0.set_random(3, normalize=False)
+    ref = O.kpm_moments(m, np.asarray(V0.to_numpy()), 8, mode=1)
+    assert relerr(K.kpm_moments(H, V0, 8, mode="two"), ref) < 1e-11

@@ -157,7 +157,8 @@ __device__ __forceinline__ void quad_fma_cols(const A (&ad)[4], const vbh_w4 (&x
 // Against a loop that waited for each batch's column index and values before
 // addressing its X rows: the 200 x 200 graphene KPM step 26.4 -> 24.5 us
 // (FP32) and 46.1 -> 38.3 us (FP64); on the 800 x 800 lattice (HBM-resident)
-// FP32 422 -> 431 us and FP64 760 -> 906 us.  Two ways to read a block's 4
+// FP32 422 -> 431 us, and FP64 760 -> 906 us with 2-slot batches or 720 us
+// with 4-slot batches (QB = 4, chosen by the launcher).  Two ways to read a block's 4
 // X rows (QX, a template parameter: a branch on it inside the batch would
 // make the compiler wait for the value loads before the X loads):
 //  * quad-contiguous (QX: one lane slice of all K columns, rows stored
@@ -171,17 +172,14 @@ __device__ __forceinline__ void quad_fma_cols(const A (&ad)[4], const vbh_w4 (&x
 // traffic is not replicated across the block's rows.  Slots past the
 // partition's count are clamped to its last slot and enter with zero A
 // values (one select per A value instead of one per product).
-template <typename T, typename V, int KT, int VN, bool QX>
+template <typename T, typename V, int KT, int VN, bool QX, int QB>
 __device__ __forceinline__ void quad_slots(const ApplyArgs<T, V>& g, int kb, int cnt, int stride, int a, int v0,
                                            int nvalid, typename Wide<T>::type (&acc)[KT]) {
     using A = typename Wide<T>::type;
     constexpr int CB = 4;                                   // slots per lane per pass
     constexpr int per = int(KT * sizeof(T) / 4);            // VGPRs of one X slice
-#ifdef VBH_QUAD_BATCH
-    constexpr int U = VBH_QUAD_BATCH;
-#else
-    constexpr int U = per <= 8 ? 4 : per <= 24 ? 2 : 1;     // slots per batch
-#endif
+    // slots per batch: QB when given, else by the slice width
+    constexpr int U = QB > 0 ? QB : per <= 8 ? 4 : per <= 24 ? 2 : 1;
     constexpr int NC = int(4 * KT * sizeof(T) / 16);        // 16-byte chunks of a quad's 4 X rows
     constexpr int NCL = (NC + 3) / 4;                       // chunks per lane
     static_assert(4 * KT * sizeof(T) % 16 == 0, "quad rows must fill whole 16-byte chunks");
@@ -273,7 +271,7 @@ __device__ __forceinline__ void quad_slots(const ApplyArgs<T, V>& g, int kb, int
 // rows; lane l works on row l % R of the segment and column slice l / R, so
 // the rows of a block row occupy consecutive lanes.  BS > 0: uniform BS x BS
 // blocks (BS == 4: quad-shared X rows, quad_slots).
-template <typename T, typename V, int BS, int S, int KT, int VN, bool DOT>
+template <typename T, typename V, int BS, int S, int KT, int VN, bool DOT, int QB = 0>
 __device__ __forceinline__ void lane_rows(const ApplyArgs<T, V>& g, int64_t sb, int64_t se, int lane, int wave,
                                           double& d1, double& d2) {
     using A = typename Wide<T>::type;
@@ -321,8 +319,8 @@ __device__ __forceinline__ void lane_rows(const ApplyArgs<T, V>& g, int64_t sb, 
             // back, 16-byte aligned (wave-uniform)
             const bool qx = S == 1 && nvalid == KT && g.ldx == KT && (reinterpret_cast<uintptr_t>(g.X) & 15) == 0;
             const int cnt = ke > kp ? (ke - kp + P - 1) / P : 0;
-            if (qx) quad_slots<T, V, KT, VN, true>(g, kp, cnt, P, a, v0, nvalid, acc);
-            else quad_slots<T, V, KT, VN, false>(g, kp, cnt, P, a, v0, nvalid, acc);
+            if (qx) quad_slots<T, V, KT, VN, true, QB>(g, kp, cnt, P, a, v0, nvalid, acc);
+            else quad_slots<T, V, KT, VN, false, QB>(g, kp, cnt, P, a, v0, nvalid, acc);
         } else if constexpr (BS > 0) {
             for (int k = kb + part; k < ke; k += P) {
                 const int col = p.col_ind[k];
@@ -390,7 +388,7 @@ __device__ __forceinline__ void lane_rows(const ApplyArgs<T, V>& g, int64_t sb, 
     }
 }
 
-template <typename T, typename V, int BS, int S, int KT, int VN, bool DOT>
+template <typename T, typename V, int BS, int S, int KT, int VN, bool DOT, int QB = 0>
 __global__ __launch_bounds__(kThreads)
 void k_apply_lanes(ApplyArgs<T, V> g, int64_t n_rows) {
     const int RPG = kWaves * (64 / (S * g.parts));
@@ -400,7 +398,7 @@ void k_apply_lanes(ApplyArgs<T, V> g, int64_t n_rows) {
     const int64_t sb = int64_t(chunk) * RPG;
     const int64_t se = sb + RPG < n_rows ? sb + RPG : n_rows;
     double d1 = 0.0, d2 = 0.0;
-    lane_rows<T, V, BS, S, KT, VN, DOT>(g, sb, se, lane, wave, d1, d2);
+    lane_rows<T, V, BS, S, KT, VN, DOT, QB>(g, sb, se, lane, wave, d1, d2);
     if constexpr (DOT) write_partials(1, lane, wave, d1, d2, g.partials + int64_t(chunk) * 2);
 }
 
@@ -1110,6 +1108,18 @@ void launch_lanes(const ApplyArgs<T, V>& a, int64_t n_rows, LaneShape sh, int bs
         with_vector<T, decltype(KT_)::value>(vn, [&](auto VN_) {
             constexpr int S = decltype(S_)::value, KT = decltype(KT_)::value, VN = decltype(VN_)::value;
             if (bs == 4) {
+                // FP64 quads of launches with one partition per row (large
+                // matrices, HBM-resident): 4-slot batches.  Measured on the
+                // 800 x 800 graphene KPM step: 720 us against 906 us with the
+                // 2-slot batches that the 200 x 200 step (two partitions)
+                // prefers (3.9 against 5.2 ms per call with 4-slot batches).
+                if constexpr (std::is_same_v<T, double> && std::is_same_v<V, double>) {
+                    if (a.parts == 1) {
+                        if (dot) hipLaunchKernelGGL((k_apply_lanes<T, V, 4, S, KT, VN, true, 4>), grid, block, 0, s, a, n_rows);
+                        else hipLaunchKernelGGL((k_apply_lanes<T, V, 4, S, KT, VN, false, 4>), grid, block, 0, s, a, n_rows);
+                        return;
+                    }
+                }
                 if (dot) hipLaunchKernelGGL((k_apply_lanes<T, V, 4, S, KT, VN, true>), grid, block, 0, s, a, n_rows);
                 else hipLaunchKernelGGL((k_apply_lanes<T, V, 4, S, KT, VN, false>), grid, block, 0, s, a, n_rows);
             } else {
