@@ -460,7 +460,7 @@ def main():
         roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK_GBS,
                     "traffic": read_traffic("pmc_kpm_step_fp32_graphene200.txt"),
-                    "kernel": "k_apply_lanes<float,float,4,1,10,2,true> (one fused KPM step)",
+                    "kernel": "k_apply_lanes<float,float,4,1,10,2,true,0> (one fused KPM step)",
                     "alg_bytes_per_launch": alg, "avg_launch_us": per_launch_ms * 1e3,
                     "note": "working set ~95 MB fits the 256 MB Infinity Cache; see spmv_large for HBM"}
         p64 = runs["fp64_fused"]
