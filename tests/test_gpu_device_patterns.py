@@ -129,3 +129,29 @@ def test_device_made_matrix_host_paths():
     assert abs(C.trace() - np.trace(D)) < 1e-9 * max(1.0, abs(np.trace(D)))
     C.shift(0.25)
     assert relerr(C.get_values(), O.shift(Cr, 0.25).vals) < 1e-12
+
+
+def test_transpose_on_device_long_columns():
+    """The transposed pattern is built on the device: column counts, a
+    scatter of the slots, a sort of each column's slots (columns longer than
+    4096 blocks go to the host sort).  6000 block rows of size 1, every row
+    holding block column 0 (a column of 6000 blocks) plus random columns;
+    against scipy's transpose (pattern and values exact)."""
+    import scipy.sparse as sp
+
+    from vbcsr_amd import workloads as W
+    from vbcsr_amd.workloads import BlockCSR
+
+    R = W.random_vbcsr(6000, bs_lo=1, bs_hi=1, per_row=6, seed=61)
+    rows = np.repeat(np.arange(R.nb), np.diff(R.row_ptr))
+    S = sp.csr_matrix((np.ones(R.nnzb), (rows, R.col_ind)), shape=(R.nb, R.nb))
+    S = (S + sp.csr_matrix((np.ones(R.nb), (np.arange(R.nb), np.zeros(R.nb, int))), shape=(R.nb, R.nb))).tocsr()
+    S.sort_indices()
+    S.data = np.random.default_rng(62).uniform(-1, 1, S.nnz)
+    A = BlockCSR(np.ones(R.nb, np.int32), S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data)
+    assert np.diff(S.tocsc().indptr).max() > 4096
+    T = mk(A).transpose()
+    ST = S.T.tocsr()
+    ST.sort_indices()
+    assert np.array_equal(T.row_ptr, ST.indptr) and np.array_equal(T.col_ind, ST.indices)
+    assert np.array_equal(np.asarray(T.get_values()), ST.data)

@@ -1097,6 +1097,35 @@ vbh_matrix* filter_by_norms(vbh_matrix* m, double thr, const double* d_norm, boo
     return g.release();
 }
 
+// Transposed pattern: blocks per block column (cnt[j] += 1 per slot of
+// column j); each slot placed in its column's segment at fill[j]++ (any
+// order: the segments are sorted afterwards); row and column of the slot
+// src[q] of every transposed entry (the row by binary search in row_ptr).
+__global__ void k_col_counts(const int* __restrict__ col_ind, int64_t nnzb, int* __restrict__ cnt) {
+    for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < nnzb; k += int64_t(gridDim.x) * blockDim.x)
+        atomicAdd(&cnt[col_ind[k]], 1);
+}
+__global__ void k_col_scatter(const int* __restrict__ col_ind, int64_t nnzb, int* __restrict__ fill,
+                              int* __restrict__ src) {
+    for (int64_t k = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; k < nnzb; k += int64_t(gridDim.x) * blockDim.x)
+        src[atomicAdd(&fill[col_ind[k]], 1)] = int(k);
+}
+__global__ void k_slot_row_col(const int* __restrict__ row_ptr, int nbr, const int* __restrict__ col_ind,
+                               const int* __restrict__ src, int64_t n, int* __restrict__ row,
+                               int* __restrict__ col) {
+    for (int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += int64_t(gridDim.x) * blockDim.x) {
+        const int k = src[q];
+        int lo = 0, hi = nbr - 1;  // last row with row_ptr[row] <= k
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (row_ptr[mid] <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        row[q] = lo;
+        col[q] = col_ind[k];
+    }
+}
+
 // Block q of the transpose is source slot src[q] (r x c) transposed (and
 // conjugated): out[b * r + a] = conj(in[a * c + b]); one wavefront per block.
 template <typename T>
@@ -1318,28 +1347,40 @@ VBH_API int vbh_matrix_transpose(vbh_matrix_t m, int conj, vbh_matrix_t* out, vo
     return guard_on(stream, [&] {
         VBH_REQUIRE(m && out, "null argument");
         hipStream_t s = (hipStream_t)stream;
-        // pattern: the column-major order of the host mirror (rows ascending
-        // within each column), source slots and their block columns
+        // pattern on the device: per-column counts, every slot placed in its
+        // column's segment, the segments sorted by slot index (slots ascend
+        // with the rows), then each entry's row and block column from its
+        // slot; the host only sees the column counts
         const int nbr = m->nbr, nbc = m->nbc;
-        std::vector<int> tp(nbc + 1, 0), tr(m->nnzb), src(m->nnzb), scol(m->nnzb);
-        const std::vector<int>& hc = host_col_ind(m, s);
-        for (int64_t k = 0; k < m->nnzb; ++k) tp[hc[k] + 1]++;
+        const int64_t nnzb = m->nnzb;
+        std::vector<int> tp(nbc + 1, 0);
+        DevList tt;
+        int* d_cnt = tt.own((int*)dalloc(sizeof(int) * size_t(std::max(nbc, 1))));
+        int* d_src = tt.own((int*)dalloc(sizeof(int) * size_t(std::max<int64_t>(nnzb, 1))));
+        int* d_tr = tt.own((int*)dalloc(sizeof(int) * size_t(std::max<int64_t>(nnzb, 1))));
+        int* d_scol = tt.own((int*)dalloc(sizeof(int) * size_t(std::max<int64_t>(nnzb, 1))));
+        const unsigned nb = unsigned(std::min<int64_t>((nnzb + 255) / 256, 8192));
+        if (nnzb > 0 && nbc > 0) {
+            VBH_CHECK_HIP(hipMemsetAsync(d_cnt, 0, sizeof(int) * size_t(nbc), s));
+            hipLaunchKernelGGL(k_col_counts, dim3(nb), dim3(256), 0, s, m->col_ind, nnzb, d_cnt);
+            VBH_CHECK_HIP(hipGetLastError());
+            VBH_CHECK_HIP(hipMemcpyAsync(tp.data() + 1, d_cnt, sizeof(int) * size_t(nbc), hipMemcpyDeviceToHost, s));
+            VBH_CHECK_HIP(hipStreamSynchronize(s));
+        }
         for (int j = 0; j < nbc; ++j) tp[j + 1] += tp[j];
-        std::vector<int> fill(tp.begin(), tp.end() - 1);
-        for (int i = 0; i < nbr; ++i)
-            for (int k = m->h_row_ptr[i]; k < m->h_row_ptr[i + 1]; ++k) {
-                const int j = hc[k];
-                const int q = fill[j]++;
-                tr[q] = i;
-                src[q] = k;
-                scol[q] = j;
-            }
-        MatrixGuard g(build_matrix(nbc, nbr, m->h_cbs.data(), m->h_rbs.data(), tp.data(), tr.data(), m->dtype, s));
+        if (nnzb > 0) {
+            int* d_tp = tt.own(upload(tp, s));
+            VBH_CHECK_HIP(hipMemcpyAsync(d_cnt, d_tp, sizeof(int) * size_t(nbc), hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(k_col_scatter, dim3(nb), dim3(256), 0, s, m->col_ind, nnzb, d_cnt, d_src);
+            VBH_CHECK_HIP(hipGetLastError());
+            sort_rows_dev(nbc, tp.data(), d_tp, d_src, s);
+            hipLaunchKernelGGL(k_slot_row_col, dim3(nb), dim3(256), 0, s, m->row_ptr, nbr, m->col_ind, d_src, nnzb,
+                               d_tr, d_scol);
+            VBH_CHECK_HIP(hipGetLastError());
+        }
+        MatrixGuard g(build_matrix_dev(nbc, nbr, m->h_cbs.data(), m->h_rbs.data(), tp.data(), d_tr, m->dtype, s));
         vbh_matrix* t = g.m;
         if (t->nnzb) {
-            DevList tmp;
-            int* d_src = tmp.own(upload(src, s));
-            int* d_scol = tmp.own(upload(scol, s));
             dispatch_dtype(m->dtype, [&](auto tag) {
                 using T = decltype(tag);
                 hipLaunchKernelGGL(k_transpose_blocks<T>, dim3(blocks_for(t->nnzb, 4)), dim3(256), 0, s,
