@@ -715,7 +715,10 @@ __device__ __forceinline__ void brw_row(const ApplyArgs<T, float>& g, const floa
     // while the FMAs of pass j run, the X rows and values of pass j+1 are in
     // flight and the entry columns of pass j+2 are being read.  Entries past
     // ne repeat the last one (in bounds) with a zero value.
-    constexpr int U = R <= 2 ? 8 : R <= 5 ? 4 : 2;
+    // (CPL >= 2: 4 entries per partition up to 9 rows and 3 above, within
+    // the 128 VGPRs of 4 waves per SIMD; against 4 up to 5 rows and 2 above,
+    // the C4 step (blocks of 1, 4, 9, 13 rows, K = 32) 497 -> 447-457 us)
+    constexpr int U = CPL >= 2 ? (R <= 2 ? 8 : R <= 9 ? 4 : 3) : (R <= 2 ? 8 : R <= 5 ? 4 : 2);
     constexpr int W = 4 * U;  // entries per pass
     auto ent = [&](int e) { return e < ne ? e : ne - 1; };
     int xr[U];             // entry columns of the next pass
