@@ -1749,46 +1749,123 @@ __global__ void k_complete_hermitian(int64_t nvals, const int64_t* __restrict__ 
     }
 }
 
+// Pattern of the Hermitian completion, on the device.  Counts: U block
+// (j, c) with c > j adds one mirrored block to row c of C (low[c]); row j
+// keeps its blocks with c >= j (own[j], a suffix of the sorted row).  One
+// wavefront per row of U.
+__global__ __launch_bounds__(256) void k_herm_counts(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                     int* __restrict__ low, int* __restrict__ own) {
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (j >= n) return;
+    int o = 0;
+    for (int k = rp[j] + lane; k < rp[j + 1]; k += 64) {
+        const int c = ci[k];
+        if (c > j) atomicAdd(&low[c], 1);
+        o += c >= j ? 1 : 0;
+    }
+    for (int d = 32; d >= 1; d >>= 1) o += __shfl_xor(o, d, 64);
+    if (lane == 0) own[j] = o;
+}
+
+// Mirrored blocks into the lower segments (lseg[fill[c]++] = U slot, any
+// order; the segments are sorted afterwards).  One wavefront per row of U.
+__global__ __launch_bounds__(256) void k_herm_scatter(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                      int* __restrict__ fill, int* __restrict__ lseg) {
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (j >= n) return;
+    for (int k = rp[j] + lane; k < rp[j + 1]; k += 64) {
+        const int c = ci[k];
+        if (c > j) lseg[atomicAdd(&fill[c], 1)] = k;
+    }
+}
+
+// Row i of C: its mirrored blocks (sorted U slots of segment lp[i].., so
+// ascending source rows; column = the source row, mode 1), then U's blocks
+// (i, c >= i) (mode 2 on the diagonal, else 0); src = U slot, crow = i.
+// One wavefront per row.
+__global__ __launch_bounds__(256) void k_herm_fill(int n, const int* __restrict__ rp, const int* __restrict__ ci,
+                                                   const int* __restrict__ crp, const int* __restrict__ lp,
+                                                   const int* __restrict__ lseg, int* __restrict__ cci,
+                                                   int* __restrict__ src, int* __restrict__ mode,
+                                                   int* __restrict__ crow) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= n) return;
+    const int nl = lp[i + 1] - lp[i];
+    for (int t = lane; t < nl; t += 64) {
+        const int k = lseg[lp[i] + t];
+        int lo = 0, hi = n - 1;  // the row of slot k
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (rp[mid] <= k) lo = mid;
+            else hi = mid - 1;
+        }
+        const int pos = crp[i] + t;
+        cci[pos] = lo;
+        src[pos] = k;
+        mode[pos] = 1;
+        crow[pos] = i;
+    }
+    const int kb = rp[i + 1] - (crp[i + 1] - crp[i] - nl);  // first block with c >= i
+    for (int k = kb + lane; k < rp[i + 1]; k += 64) {
+        const int pos = crp[i] + nl + (k - kb);
+        const int c = ci[k];
+        cci[pos] = c;
+        src[pos] = k;
+        mode[pos] = c == i ? 2 : 0;
+        crow[pos] = i;
+    }
+}
+
 static void complete_hermitian_impl(vbh_matrix_t U, vbh_matrix_t* out, hipStream_t s) {
     VBH_REQUIRE(U && out, "null argument");
     VBH_REQUIRE(U->nbr == U->nbc, "Hermitian completion needs a square pattern without ghosts");
     const int n = U->nbr;
     for (int i = 0; i < n; ++i)
         VBH_REQUIRE(U->h_rbs[i] == U->h_cbs[i], "Hermitian completion needs equal row and column block sizes");
-    const std::vector<int>& rp = U->h_row_ptr;
-    const std::vector<int>& ci = host_col_ind(U, s);
     // rows of C: mirrored blocks (column j < i, from U's row j) then U's own
-    // blocks with j >= i; U blocks below the diagonal are ignored
-    std::vector<std::vector<std::pair<int, int>>> lower(n);  // (col, U slot)
-    for (int j = 0; j < n; ++j)
-        for (int k = rp[j]; k < rp[j + 1]; ++k)
-            if (ci[k] > j) lower[ci[k]].push_back({j, k});
-    std::vector<int> crp(n + 1, 0), cci, src, mode;
-    for (int i = 0; i < n; ++i) {
-        std::sort(lower[i].begin(), lower[i].end());
-        for (auto& q : lower[i]) {
-            cci.push_back(q.first);
-            src.push_back(q.second);
-            mode.push_back(1);
-        }
-        for (int k = rp[i]; k < rp[i + 1]; ++k) {
-            if (ci[k] < i) continue;
-            cci.push_back(ci[k]);
-            src.push_back(k);
-            mode.push_back(ci[k] == i ? 2 : 0);
-        }
-        crp[i + 1] = int(cci.size());
+    // blocks with j >= i; U blocks below the diagonal are ignored.  The
+    // pattern is built on the device; the host receives the row counts.
+    std::vector<int> low(n), own(n), crp(n + 1, 0), lp(n + 1, 0);
+    DevList tmp;
+    int* d_low = tmp.own((int*)dalloc(sizeof(int) * size_t(std::max(n, 1))));
+    int* d_own = tmp.own((int*)dalloc(sizeof(int) * size_t(std::max(n, 1))));
+    const unsigned nb4 = unsigned((n + 3) / 4);
+    if (n > 0) {
+        VBH_CHECK_HIP(hipMemsetAsync(d_low, 0, sizeof(int) * size_t(n), s));
+        hipLaunchKernelGGL(k_herm_counts, dim3(nb4), dim3(256), 0, s, n, U->row_ptr, U->col_ind, d_low, d_own);
+        VBH_CHECK_HIP(hipGetLastError());
+        VBH_CHECK_HIP(hipMemcpyAsync(low.data(), d_low, sizeof(int) * size_t(n), hipMemcpyDeviceToHost, s));
+        VBH_CHECK_HIP(hipMemcpyAsync(own.data(), d_own, sizeof(int) * size_t(n), hipMemcpyDeviceToHost, s));
+        VBH_CHECK_HIP(hipStreamSynchronize(s));
     }
-    vbh_matrix* C = build_matrix(n, n, U->h_rbs.data(), U->h_cbs.data(), crp.data(), cci.data(), U->dtype, s);
+    for (int i = 0; i < n; ++i) {
+        VBH_REQUIRE(int64_t(crp[i]) + low[i] + own[i] <= int64_t(INT32_MAX), "Hermitian completion exceeds 2^31 blocks");
+        crp[i + 1] = crp[i] + low[i] + own[i];
+        lp[i + 1] = lp[i] + low[i];
+    }
+    const int64_t total = crp[n];
+    int* d_cci = tmp.own((int*)dalloc(sizeof(int) * size_t(std::max<int64_t>(total, 1))));
+    int* d_src = tmp.own((int*)dalloc(sizeof(int) * size_t(std::max<int64_t>(total, 1))));
+    int* d_mode = tmp.own((int*)dalloc(sizeof(int) * size_t(std::max<int64_t>(total, 1))));
+    int* d_crow = tmp.own((int*)dalloc(sizeof(int) * size_t(std::max<int64_t>(total, 1))));
+    if (total > 0) {
+        int* d_crp = tmp.own(upload(crp, s));
+        int* d_lp = tmp.own(upload(lp, s));
+        int* d_lseg = tmp.own((int*)dalloc(sizeof(int) * size_t(std::max(lp[n], 1))));
+        VBH_CHECK_HIP(hipMemcpyAsync(d_low, d_lp, sizeof(int) * size_t(n), hipMemcpyDeviceToDevice, s));
+        hipLaunchKernelGGL(k_herm_scatter, dim3(nb4), dim3(256), 0, s, n, U->row_ptr, U->col_ind, d_low, d_lseg);
+        VBH_CHECK_HIP(hipGetLastError());
+        sort_rows_dev(n, lp.data(), d_lp, d_lseg, s);
+        hipLaunchKernelGGL(k_herm_fill, dim3(nb4), dim3(256), 0, s, n, U->row_ptr, U->col_ind, d_crp, d_lp, d_lseg,
+                           d_cci, d_src, d_mode, d_crow);
+        VBH_CHECK_HIP(hipGetLastError());
+    }
+    vbh_matrix* C = build_matrix_dev(n, n, U->h_rbs.data(), U->h_cbs.data(), crp.data(), d_cci, U->dtype, s);
     try {
         if (C->nnz > 0) {
-            std::vector<int> crow(C->nnzb);
-            for (int i = 0; i < n; ++i)
-                for (int k = crp[i]; k < crp[i + 1]; ++k) crow[k] = i;
-            DevList tmp;
-            int* d_src = tmp.own(upload(src, s));
-            int* d_mode = tmp.own(upload(mode, s));
-            int* d_crow = tmp.own(upload(crow, s));
             const unsigned nb = unsigned((C->nnz + 255) / 256);
             if (U->dtype == VBH_F64)
                 hipLaunchKernelGGL(k_complete_hermitian<double>, dim3(nb), dim3(256), 0, s, C->nnz, C->voff,
