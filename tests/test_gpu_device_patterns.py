@@ -155,3 +155,33 @@ def test_transpose_on_device_long_columns():
     ST.sort_indices()
     assert np.array_equal(T.row_ptr, ST.indptr) and np.array_equal(T.col_ind, ST.indices)
     assert np.array_equal(np.asarray(T.get_values()), ST.data)
+
+
+def test_hermitian_completion_long_mirrored_rows():
+    """The Hermitian completion's pattern is built on the device; a row of C
+    that mirrors more than 4096 blocks takes the host sort.  6000 block rows
+    of size 1: an upper pattern with the diagonal, random columns and the
+    last column in every row (so C's last row mirrors 5999 blocks), against
+    scipy's triu(U, 1) + triu(U, 1)^T + diag(U) (pattern and values exact)."""
+    import scipy.sparse as sp
+
+    from vbcsr_amd import workloads as W
+    from vbcsr_amd.workloads import BlockCSR
+
+    n = 6000
+    R = W.random_vbcsr(n, bs_lo=1, bs_hi=1, per_row=4, seed=72)
+    rows = np.concatenate([np.repeat(np.arange(n), np.diff(R.row_ptr)), np.arange(n)])
+    cols = np.concatenate([R.col_ind, np.full(n, n - 1)])
+    keep = cols >= rows
+    S = sp.csr_matrix((np.ones(int(keep.sum())), (rows[keep], cols[keep])), shape=(n, n))
+    S.sum_duplicates()
+    S.sort_indices()
+    S.data = np.random.default_rng(73).uniform(-1, 1, S.nnz)
+    U = BlockCSR(np.ones(n, np.int32), S.indptr.astype(np.int32), S.indices.astype(np.int32), S.data)
+    C = mk(U).complete_hermitian()
+    up = sp.triu(S, 1)
+    E = (up + up.T + sp.diags(S.diagonal())).tocsr()
+    E.sort_indices()
+    assert np.diff(E.indptr).max() > 4096
+    assert np.array_equal(C.row_ptr, E.indptr) and np.array_equal(C.col_ind, E.indices)
+    assert np.array_equal(np.asarray(C.get_values()), E.data)
