@@ -1077,7 +1077,9 @@ __global__ __launch_bounds__(256) void k_densify(const T* __restrict__ vals, con
 // multiplied, and one barrier separates the steps.  Measured on a 30k x 30k
 // x 30k product: 128 x 128 tiles with k-steps of 32, 114 ms; 256 x 256, 59
 // ms (half the operand bytes per flop); k-steps of 64, 45 ms (half the
-// barriers); grouped tile order, 43.6 ms (1.26 PFLOP/s).
+// barriers); grouped tile order, 43.6 ms (1.26 PFLOP/s).  Larger wave parts
+// (fewer waves, fewer LDS reads per MFMA) were slower: 128 x 64, 53.6 ms;
+// 64 x 128, 49.5 ms; 128 x 128, 70.7 ms.
 #ifndef VBH_GEMM_TM
 #define VBH_GEMM_TM 256
 #endif
@@ -1095,8 +1097,16 @@ constexpr int kGemmTN = VBH_GEMM_TN;
 constexpr int kGemmK = VBH_GEMM_K;
 constexpr int kGemmPitch = kGemmK + 8;
 constexpr int kGemmGroup = VBH_GEMM_GROUP;
-constexpr int kGemmWaves = (kGemmTM / 64) * (kGemmTN / 64);
-static_assert(kGemmTM % 64 == 0 && kGemmTN % 64 == 0 && kGemmWaves <= 16, "work-group tile of 64 x 64 wave tiles");
+#ifndef VBH_GEMM_WTM
+#define VBH_GEMM_WTM 64
+#endif
+#ifndef VBH_GEMM_WTN
+#define VBH_GEMM_WTN 64
+#endif
+constexpr int kGemmWTM = VBH_GEMM_WTM, kGemmWTN = VBH_GEMM_WTN;  // one wave's part of the tile
+constexpr int kGemmFI = kGemmWTM / 16, kGemmFJ = kGemmWTN / 16;  // its 16 x 16 MFMA tiles
+constexpr int kGemmWaves = (kGemmTM / kGemmWTM) * (kGemmTN / kGemmWTN);
+static_assert(kGemmTM % kGemmWTM == 0 && kGemmTN % kGemmWTN == 0 && kGemmWaves <= 16, "work-group tile of wave tiles");
 
 __global__ __launch_bounds__(64 * kGemmWaves) void k_gemm_bf16_nt(const short* __restrict__ A,
                                                                   const short* __restrict__ BT,
@@ -1114,7 +1124,7 @@ __global__ __launch_bounds__(64 * kGemmWaves) void k_gemm_bf16_nt(const short* _
     const int gm = tiles_m - grp * kGemmGroup < kGemmGroup ? tiles_m - grp * kGemmGroup : kGemmGroup;
     const int tm = grp * kGemmGroup + q % gm, tn = q / gm;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave / (kGemmTN / 64), wn = wave % (kGemmTN / 64);
+    const int wm = wave / (kGemmTN / kGemmWTN), wn = wave % (kGemmTN / kGemmWTN);
     const int64_t m0 = int64_t(tm) * kGemmTM, n0 = int64_t(tn) * kGemmTN;
     // staging: RC 16-byte chunks per operand row and k-step (chunk c: row
     // c / RC, k-offset 8 * (c % RC)), CA / CB per thread
@@ -1147,11 +1157,11 @@ __global__ __launch_bounds__(64 * kGemmWaves) void k_gemm_bf16_nt(const short* _
             *reinterpret_cast<i32x4*>(&bs[buf][(c / RC) * kGemmPitch + (c % RC) * 8]) = rb[u];
         }
     };
-    f32x4 acc[4][4];
+    f32x4 acc[kGemmFI][kGemmFJ];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < kGemmFI; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int j = 0; j < kGemmFJ; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     const int lr = lane & 15, lk = (lane >> 4) * 8;
     const int nsteps = int(Kp / kGemmK);
     load(0);
@@ -1162,17 +1172,17 @@ __global__ __launch_bounds__(64 * kGemmWaves) void k_gemm_bf16_nt(const short* _
         if (st + 1 < nsteps) load(int64_t(st + 1) * kGemmK);
 #pragma unroll
         for (int kk = 0; kk < kGemmK; kk += 32) {
-            bf16x8 av[4], bv[4];
+            bf16x8 av[kGemmFI], bv[kGemmFJ];
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
-                av[i] = *reinterpret_cast<const bf16x8*>(&as[buf][(wm * 64 + i * 16 + lr) * kGemmPitch + kk + lk]);
+            for (int i = 0; i < kGemmFI; ++i)
+                av[i] = *reinterpret_cast<const bf16x8*>(&as[buf][(wm * kGemmWTM + i * 16 + lr) * kGemmPitch + kk + lk]);
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
-                bv[j] = *reinterpret_cast<const bf16x8*>(&bs[buf][(wn * 64 + j * 16 + lr) * kGemmPitch + kk + lk]);
+            for (int j = 0; j < kGemmFJ; ++j)
+                bv[j] = *reinterpret_cast<const bf16x8*>(&bs[buf][(wn * kGemmWTN + j * 16 + lr) * kGemmPitch + kk + lk]);
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < kGemmFI; ++i)
 #pragma unroll
-                for (int j = 0; j < 4; ++j)
+                for (int j = 0; j < kGemmFJ; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[i], bv[j], acc[i][j], 0, 0, 0);
         }
         if (st + 1 < nsteps) store(buf ^ 1);
@@ -1180,13 +1190,13 @@ __global__ __launch_bounds__(64 * kGemmWaves) void k_gemm_bf16_nt(const short* _
     }
     // result lane l: column l % 16, rows 4 (l / 16) .. +3 of each 16 x 16 tile
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < kGemmFI; ++i)
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < kGemmFJ; ++j)
 #pragma unroll
             for (int e = 0; e < 4; ++e) {
-                const int64_t row = m0 + wm * 64 + i * 16 + (lane >> 4) * 4 + e;
-                const int64_t col = n0 + wn * 64 + j * 16 + lr;
+                const int64_t row = m0 + wm * kGemmWTM + i * 16 + (lane >> 4) * 4 + e;
+                const int64_t col = n0 + wn * kGemmWTN + j * 16 + lr;
                 C[row * ldc + col] = acc[i][j][e];
             }
 }
