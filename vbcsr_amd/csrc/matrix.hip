@@ -5,6 +5,7 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <numeric>
@@ -26,9 +27,76 @@ size_t dtype_size(int dtype) {
     throw Error(VBH_ERR_ARG, "unknown dtype");
 }
 
+// Large device buffers (>= 64 MiB) are kept for reuse when freed, per
+// device, up to 16 GiB: releasing a multi-GB buffer took 130-175 ms here
+// (the two largest frees of a late McWeeny iteration), and products and
+// sums allocate buffers of similar sizes again.  A kept buffer is handed out
+// for a request of at least 4/5 of its size.  dfree synchronises the device
+// before keeping a buffer, as hipFree does, so no queued kernel still uses
+// it when it is handed out again; a failed allocation releases the kept
+// buffers and retries.
+namespace {
+struct KeptBlock {
+    void* p;
+    size_t bytes;
+    int dev;
+};
+constexpr size_t kKeepMin = size_t(64) << 20;
+constexpr size_t kKeepCap = size_t(16) << 30;
+std::mutex g_keep_mu;
+std::vector<KeptBlock> g_kept;                      // free, kept for reuse
+std::map<void*, std::pair<size_t, int>> g_large;     // live large blocks: size, device
+size_t g_kept_bytes = 0;
+
+void release_kept(int dev) {  // g_keep_mu held
+    for (size_t q = 0; q < g_kept.size();) {
+        if (g_kept[q].dev == dev) {
+            (void)hipFree(g_kept[q].p);
+            g_kept_bytes -= g_kept[q].bytes;
+            g_kept[q] = g_kept.back();
+            g_kept.pop_back();
+        } else {
+            ++q;
+        }
+    }
+}
+}  // namespace
+
 void* dalloc(size_t bytes) {
     void* p = nullptr;
     if (bytes == 0) bytes = 16;
+    if (bytes >= kKeepMin) {
+        int dev = 0;
+        VBH_CHECK_HIP(hipGetDevice(&dev));
+        std::lock_guard<std::mutex> lk(g_keep_mu);
+        size_t best = SIZE_MAX, bq = 0;
+        for (size_t q = 0; q < g_kept.size(); ++q)
+            if (g_kept[q].dev == dev && g_kept[q].bytes >= bytes && g_kept[q].bytes / 5 * 4 <= bytes &&
+                g_kept[q].bytes < best) {
+                best = g_kept[q].bytes;
+                bq = q;
+            }
+        if (best != SIZE_MAX) {
+            p = g_kept[bq].p;
+            g_kept_bytes -= best;
+            g_kept[bq] = g_kept.back();
+            g_kept.pop_back();
+            g_large[p] = {best, dev};
+            return p;
+        }
+        hipError_t e = hipMalloc(&p, bytes);
+        if (e != hipSuccess && !g_kept.empty()) {
+            (void)hipGetLastError();
+            release_kept(dev);
+            e = hipMalloc(&p, bytes);
+        }
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(VBH_ERR_NOMEM, "hipMalloc(" + std::to_string(bytes) + ") failed: " + hipGetErrorString(e));
+        }
+        g_large[p] = {bytes, dev};
+        return p;
+    }
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
         (void)hipGetLastError();
@@ -38,7 +106,25 @@ void* dalloc(size_t bytes) {
 }
 
 void dfree(void* p) {
-    if (p) (void)hipFree(p);
+    if (!p) return;
+    {
+        std::lock_guard<std::mutex> lk(g_keep_mu);
+        auto it = g_large.find(p);
+        if (it != g_large.end()) {
+            const size_t bytes = it->second.first;
+            const int dev = it->second.second;
+            g_large.erase(it);
+            int cur = -1;
+            if (hipGetDevice(&cur) == hipSuccess && cur == dev && g_kept_bytes + bytes <= kKeepCap &&
+                hipDeviceSynchronize() == hipSuccess) {
+                g_kept.push_back({p, bytes, dev});
+                g_kept_bytes += bytes;
+                return;
+            }
+            (void)hipGetLastError();
+        }
+    }
+    (void)hipFree(p);
 }
 
 // ---------------------------------------------------------------- stream state
