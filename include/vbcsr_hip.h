@@ -66,6 +66,9 @@ VBH_API const char* vbh_last_error(void);
 VBH_API int vbh_version(void);
 VBH_API int vbh_device_count(int* count);
 VBH_API int vbh_set_device(int device);
+/* Device memory of the library's allocator.  Buffers of 64 MiB and more are
+ * kept for reuse when freed (per device, up to 16 GiB in total; released when
+ * an allocation fails); vbh_free synchronises the device, as hipFree does. */
 VBH_API int vbh_malloc(void** ptr, size_t bytes);
 VBH_API int vbh_free(void* ptr);
 VBH_API int vbh_memcpy(void* dst, const void* src, size_t bytes, int kind /*0 h2d 1 d2h 2 d2d*/,
