@@ -1051,26 +1051,38 @@ __device__ inline CT to_dense(double x) {
 // transposed; one wavefront per stored block (out zeroed first).
 template <typename T, typename CT>
 __global__ __launch_bounds__(256) void k_densify(const T* __restrict__ vals, const int64_t* __restrict__ voff,
-                                                 const int* __restrict__ row_ptr, const int* __restrict__ col_ind,
+                                                 const int* __restrict__ srow, const int* __restrict__ col_ind,
                                                  const int* __restrict__ roff, const int* __restrict__ coff,
-                                                 const int* __restrict__ cbs, int nbr, int64_t nnzb, int64_t ld,
+                                                 const int* __restrict__ cbs, int64_t nnzb, int64_t ld,
                                                  int transposed, CT* __restrict__ out) {
     const int64_t k = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (k >= nnzb) return;
-    int lo = 0, hi = nbr - 1;  // block row of slot k: last i with row_ptr[i] <= k
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (row_ptr[mid] <= k) lo = mid;
-        else hi = mid - 1;
-    }
-    const int j = col_ind[k], c = cbs[j];
+    const int i = srow[k], j = col_ind[k], c = cbs[j];
     const int64_t n = voff[k + 1] - voff[k];
+    const int r = int(n / c);
     for (int64_t e = lane; e < n; e += 64) {
-        const int a = int(e / c), b = int(e - int64_t(a) * c);
-        const int64_t row = int64_t(roff[lo]) + a, col = int64_t(coff[j]) + b;
-        out[transposed ? col * ld + row : row * ld + col] = to_dense<CT>(double(vals[voff[k] + e]));
+        // consecutive lanes write consecutive output elements: along the
+        // block's row (row-major output) or down its column (transposed)
+        int a, b;
+        if (transposed) {
+            b = int(e / r);
+            a = int(e - int64_t(b) * r);
+        } else {
+            a = int(e / c);
+            b = int(e - int64_t(a) * c);
+        }
+        const int64_t row = int64_t(roff[i]) + a, col = int64_t(coff[j]) + b;
+        out[transposed ? col * ld + row : row * ld + col] = to_dense<CT>(double(vals[voff[k] + int64_t(a) * c + b]));
     }
+}
+
+// srow[k] = block row of slot k.  One wavefront per block row.
+__global__ __launch_bounds__(256) void k_slot_rows(const int* __restrict__ row_ptr, int nbr, int* __restrict__ srow) {
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (i >= nbr) return;
+    for (int k = row_ptr[i] + lane; k < row_ptr[i + 1]; k += 64) srow[k] = i;
 }
 
 // ------------------------------------------------------------ BF16 dense products
@@ -1212,20 +1224,15 @@ __global__ __launch_bounds__(64 * kGemmWaves) void k_gemm_bf16_nt(const short* _
 // C's blocks (its symbolic pattern) from the dense FP32 product, widened to
 // FP64, with each block's squared Frobenius norm; one wavefront per block.
 __global__ __launch_bounds__(256) void k_scatter_dense(const float* __restrict__ Cd, int64_t ldc,
-                                                       const int* __restrict__ row_ptr, const int* __restrict__ col_ind,
+                                                       const int* __restrict__ srow, const int* __restrict__ col_ind,
                                                        const int64_t* __restrict__ voff, const int* __restrict__ roff,
                                                        const int* __restrict__ coff, const int* __restrict__ cbs,
-                                                       int nbr, int64_t nnzb, double* __restrict__ vals,
+                                                       int64_t nnzb, double* __restrict__ vals,
                                                        double* __restrict__ norm2) {
     const int64_t k = int64_t(blockIdx.x) * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (k >= nnzb) return;
-    int lo = 0, hi = nbr - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (row_ptr[mid] <= k) lo = mid;
-        else hi = mid - 1;
-    }
+    const int lo = srow[k];
     const int j = col_ind[k], c = cbs[j];
     const int64_t n = voff[k + 1] - voff[k];
     double s = 0.0;
@@ -1523,8 +1530,20 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
             VBH_CHECK_HIP(hipMemGetInfo(&free_b, &total_b));
             dense16 = need <= free_b / 2;
         }
+        // block row of every slot, for the dense copies and the scatter
+        // (one array per matrix, A's reused when B is A)
+        auto slot_rows = [&](const vbh_matrix* M) {
+            int* d = tmp.own((int*)dalloc(sizeof(int) * size_t(std::max<int64_t>(M->nnzb, 1))));
+            if (M->nbr > 0)
+                hipLaunchKernelGGL(k_slot_rows, dim3(nblk4(M->nbr)), dim3(256), 0, s, M->row_ptr, M->nbr, d);
+            VBH_CHECK_HIP(hipGetLastError());
+            return d;
+        };
         if (dense16) {
             d_cnorm2 = tmp.own((double*)dalloc(sizeof(double) * size_t(Cm->nnzb)));
+            const int* a_srow = slot_rows(A);
+            const int* b_srow = B == A ? a_srow : slot_rows(B);
+            const int* c_srow = slot_rows(Cm);
             DevList dt;
             short* a16 = dt.own((short*)dalloc(sizeof(short) * size_t(Mp) * size_t(Kp)));
             short* b16 = dt.own((short*)dalloc(sizeof(short) * size_t(Np) * size_t(Kp)));
@@ -1533,20 +1552,20 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
             VBH_CHECK_HIP(hipMemsetAsync(b16, 0, sizeof(short) * size_t(Np) * size_t(Kp), s));
             if (A->nnzb)
                 hipLaunchKernelGGL((k_densify<double, short>), dim3(nblk4(A->nnzb)), dim3(256), 0, s,
-                                   (const double*)A->vals, A->voff, A->row_ptr, A->col_ind, A->roff, A->coff, A->cbs,
-                                   A->nbr, A->nnzb, Kp, 0, a16);
+                                   (const double*)A->vals, A->voff, a_srow, A->col_ind, A->roff, A->coff, A->cbs,
+                                   A->nnzb, Kp, 0, a16);
             if (B->nnzb)
                 hipLaunchKernelGGL((k_densify<double, short>), dim3(nblk4(B->nnzb)), dim3(256), 0, s,
-                                   (const double*)B->vals, B->voff, B->row_ptr, B->col_ind, B->roff, B->coff, B->cbs,
-                                   B->nbr, B->nnzb, Kp, 1, b16);
+                                   (const double*)B->vals, B->voff, b_srow, B->col_ind, B->roff, B->coff, B->cbs,
+                                   B->nnzb, Kp, 1, b16);
             VBH_CHECK_HIP(hipGetLastError());
             const int tiles_n = int(Np / kGemmTN);
             const int n_tiles = int(Mp / kGemmTM) * tiles_n;
             hipLaunchKernelGGL(k_gemm_bf16_nt, dim3((n_tiles + 7) & ~7), dim3(64 * kGemmWaves), 0, s, a16, b16, cd,
                                Kp, Np, tiles_n, n_tiles);
             VBH_CHECK_HIP(hipGetLastError());
-            hipLaunchKernelGGL(k_scatter_dense, dim3(nblk4(Cm->nnzb)), dim3(256), 0, s, cd, Np, Cm->row_ptr,
-                               Cm->col_ind, Cm->voff, Cm->roff, Cm->coff, Cm->cbs, nbr, Cm->nnzb, (double*)Cm->vals,
+            hipLaunchKernelGGL(k_scatter_dense, dim3(nblk4(Cm->nnzb)), dim3(256), 0, s, cd, Np, c_srow,
+                               Cm->col_ind, Cm->voff, Cm->roff, Cm->coff, Cm->cbs, Cm->nnzb, (double*)Cm->vals,
                                d_cnorm2);
             VBH_CHECK_HIP(hipGetLastError());
             VBH_CHECK_HIP(hipStreamSynchronize(s));  // before the temporaries are freed
@@ -1561,14 +1580,16 @@ static void spgemm_impl(vbh_matrix_t A, vbh_matrix_t B, double thr, int numeric,
                 CT* bd = pt.own((CT*)dalloc(sizeof(CT) * size_t(B->n_rows) * size_t(B->n_cols)));
                 VBH_CHECK_HIP(hipMemsetAsync(ad, 0, sizeof(CT) * size_t(A->n_rows) * size_t(A->n_cols), s));
                 VBH_CHECK_HIP(hipMemsetAsync(bd, 0, sizeof(CT) * size_t(B->n_rows) * size_t(B->n_cols), s));
+                const int* a_srow = slot_rows(A);
+                const int* b_srow = B == A ? a_srow : slot_rows(B);
                 if (A->nnzb)
                     hipLaunchKernelGGL((k_densify<double, CT>), dim3(nblk4(A->nnzb)), dim3(256), 0, s,
-                                       (const double*)A->vals, A->voff, A->row_ptr, A->col_ind, A->roff, A->coff,
-                                       A->cbs, A->nbr, A->nnzb, int64_t(A->n_cols), 0, ad);
+                                       (const double*)A->vals, A->voff, a_srow, A->col_ind, A->roff, A->coff,
+                                       A->cbs, A->nnzb, int64_t(A->n_cols), 0, ad);
                 if (B->nnzb)
                     hipLaunchKernelGGL((k_densify<double, CT>), dim3(nblk4(B->nnzb)), dim3(256), 0, s,
-                                       (const double*)B->vals, B->voff, B->row_ptr, B->col_ind, B->roff, B->coff,
-                                       B->cbs, B->nbr, B->nnzb, int64_t(B->n_cols), 0, bd);
+                                       (const double*)B->vals, B->voff, b_srow, B->col_ind, B->roff, B->coff,
+                                       B->cbs, B->nnzb, int64_t(B->n_cols), 0, bd);
                 VBH_CHECK_HIP(hipGetLastError());
                 double* anm = pt.own((double*)dalloc(sizeof(double) * size_t(nbr) * size_t(A->nbc)));
                 double* bnm = pt.own((double*)dalloc(sizeof(double) * size_t(B->nbr) * size_t(B->nbc)));
