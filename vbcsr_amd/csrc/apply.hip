@@ -106,11 +106,10 @@ __global__ __launch_bounds__(kThreads) void k_apply_adjoint(AdjointArgs<T> g) {
                 typename Wide<T>::type acc = Sc<typename Wide<T>::type>::zero();
                 if (active) {
                     for (int q = qb + p_l; q < qe; q += P) {
-                        const int i = g.t_row[q];
-                        const int k = g.t_slot[q];
-                        const int r = p.rbs[i];
-                        const T* blk = g.vals + p.voff[k] + b;
-                        const T* xr = g.X + int64_t(p.roff[i]) * g.ldx + v;
+                        const int2 rr = g.t_rows[q];  // (first scalar row, rows) of the block row
+                        const int r = rr.y;
+                        const T* blk = g.vals + g.t_voff[q] + b;
+                        const T* xr = g.X + int64_t(rr.x) * g.ldx + v;
                         for (int a = 0; a < r; ++a)
                             acc = Sc<typename Wide<T>::type>::fma(widen(Sc<T>::conj(blk[int64_t(a) * c])), widen(xr[int64_t(a) * g.ldx]), acc);
                     }
@@ -264,8 +263,8 @@ void apply_adjoint(vbh_matrix* m, const T* vals, int K, const void* X, int64_t l
     a.Y = (T*)Y;
     a.ldy = ldy;
     a.t_ptr = m->t_ptr;
-    a.t_row = m->t_row;
-    a.t_slot = m->t_slot;
+    a.t_voff = m->t_voff;
+    a.t_rows = m->t_rows;
     a.split = m->t_split;
     a.G = m->tG;
     hipLaunchKernelGGL(k_apply_adjoint<T>, dim3(a.G), dim3(kThreads), 0, s, a);

@@ -66,8 +66,8 @@ struct AdjointArgs {
     T* Y;
     int64_t ldy;
     const int* t_ptr;
-    const int* t_row;
-    const int* t_slot;
+    const int64_t* t_voff;
+    const int2* t_rows;
     const int* split;
     int G;
 };

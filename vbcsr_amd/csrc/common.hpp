@@ -272,7 +272,11 @@ struct vbh_matrix {
     int* split = nullptr;
     int G = 0;
     // adjoint plan (CSC of the pattern), built on demand
-    int *t_ptr = nullptr, *t_row = nullptr, *t_slot = nullptr, *t_split = nullptr;
+    // incoming block q of column j (t_ptr[j] <= q < t_ptr[j+1]): its value
+    // offset and (first scalar row, row count) of its block row
+    int *t_ptr = nullptr, *t_split = nullptr;
+    int64_t* t_voff = nullptr;
+    int2* t_rows = nullptr;
     int tG = 0;
     bool t_valid = false;
     // Guards the structures built on demand (FP32 copy, row-class lists,

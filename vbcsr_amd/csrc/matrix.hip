@@ -899,7 +899,7 @@ void destroy_matrix(vbh_matrix* m) {
     for (void* p : {(void*)m->rbs, (void*)m->cbs, (void*)m->row_ptr, (void*)m->col_ind, (void*)m->roff,
                     (void*)m->coff, (void*)m->voff, (void*)m->meta, (void*)m->row_of, m->vals, m->vals32,
                     (void*)m->split, (void*)m->t_ptr, (void*)m->vt, (void*)m->eoff, (void*)m->ecol,
-                    (void*)m->t_row, (void*)m->t_slot, (void*)m->t_split})
+                    (void*)m->t_voff, (void*)m->t_rows, (void*)m->t_split})
         dfree(p);
     for (auto& kv : m->row_lists) {
         dfree(kv.second.small_rows);
@@ -1059,9 +1059,18 @@ void ensure_adjoint_plan(vbh_matrix* m, hipStream_t s) {
         while (col < m->nbc && work[col] < target) ++col;
         split[g] = col;
     }
+    // per incoming block: its value offset and its block row's (first
+    // scalar row, row count), so the adjoint kernel reads them in one step
+    const std::vector<int64_t>& hv = host_voff(m, s);
+    std::vector<int64_t> tvo(m->nnzb);
+    std::vector<int2> trr(m->nnzb);
+    for (int64_t q = 0; q < m->nnzb; ++q) {
+        tvo[q] = hv[ts[q]];
+        trr[q] = make_int2(m->h_roff[tr[q]], m->h_rbs[tr[q]]);
+    }
     m->t_ptr = upload(tp, s);
-    m->t_row = upload(tr, s);
-    m->t_slot = upload(ts, s);
+    m->t_voff = upload(tvo, s);
+    m->t_rows = upload(trr, s);
     m->t_split = upload(split, s);
     m->tG = G;
     VBH_CHECK_HIP(hipStreamSynchronize(s));
